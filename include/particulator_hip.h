@@ -1,0 +1,261 @@
+/*
+ * particulator_hip.h -- C ABI of the MI355X (gfx950) null-collision Monte Carlo step.
+ *
+ * The library advances electron / photon / positron populations in applied E/B fields:
+ * relativistic RK2 push, null-collision selection from a binary-interval Chebyshev rate
+ * table, per-process secondary sampling, append of secondaries and compaction of the
+ * populations.  Host code (Julia `ccall`, Python `ctypes`, C/C++) owns table building and
+ * the run loop; the library owns device memory.
+ *
+ * Conventions: every entry point returns PT_OK (0) or a negative pt_status; the message of
+ * the last failure is pt_last_error(ctx).  No C++ exception crosses this boundary.  A
+ * context is used from one host thread at a time.  Calls are ordered on the context's HIP
+ * stream and complete before they return unless noted.  Units are SI (J, s, m, kg m/s).
+ *
+ * Each entry point names the reference function (Particulator.jl, file:line) whose role
+ * it takes over.
+ */
+#ifndef PARTICULATOR_HIP_H
+#define PARTICULATOR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_ABI_VERSION 1
+
+/* particle species: the MultiPopulation order of the reference scripts
+ * (scripts/swarm.jl:68-71) */
+enum pt_species { PT_ELECTRON = 0, PT_PHOTON = 1, PT_POSITRON = 2, PT_NSPECIES = 3 };
+
+/* collision processes (the `collide` methods of src/rbeb.jl, src/moller.jl, src/seltzer.jl,
+ * src/relativistic_coulomb.jl, src/compton.jl, src/photo_electric.jl,
+ * src/bethe_heitler.jl, src/bhaba.jl, src/anihilation.jl) */
+enum pt_proc_kind {
+    PT_PROC_NULL = 0,
+    PT_PROC_ELASTIC = 1,        /* RelativisticCoulomb(Z) */
+    PT_PROC_SB = 2,             /* SeltzerBerger(Z) */
+    PT_PROC_RBEB = 3,           /* RBEB(B, U, N) */
+    PT_PROC_MOLLER = 4,         /* Moller(Z, tcut) */
+    PT_PROC_COMPTON = 5,        /* Compton(Z) */
+    PT_PROC_PHOTOELECTRIC = 6,  /* PhotoElectric(Z) */
+    PT_PROC_BETHE_HEITLER = 7,  /* BetheHeitler(Z) */
+    PT_PROC_BHABHA = 8,         /* Bhaba(Z, tcut) */
+    PT_PROC_ANNIHILATION = 9,   /* PositronAnihilation(Z) */
+    PT_PROC_KN_COMPTON = 10     /* KleinNishinaCompton(Z) */
+};
+
+/* field functors, src/field.jl:4-52 */
+enum pt_field_kind {
+    PT_FIELD_NONE = 0, PT_FIELD_HOMOGENEOUS = 1, PT_FIELD_DOUBLELAYER = 2,
+    PT_FIELD_STEP = 3, PT_FIELD_CONFINED = 4
+};
+
+enum pt_status {
+    PT_OK = 0,
+    PT_E_ARG = -1,          /* invalid argument */
+    PT_E_HIP = -2,          /* HIP runtime error */
+    PT_E_CAPACITY = -3,     /* a population reached its capacity (src/population.jl:107) */
+    PT_E_RATEBOUND = -4,    /* rate bound underestimated (src/collisions.jl:186) */
+    PT_E_STATE = -5,        /* call out of order (e.g. no table uploaded) */
+    PT_E_NOMEM = -6,        /* device allocation failed */
+    PT_E_TABLE_RANGE = -7,  /* energy above the table's xmax (src/cheby.jl:127-143) */
+    PT_E_SAMPLER = -8,      /* a sampler's domain assertion failed (e.g. src/seltzer.jl:73) */
+    PT_E_NOT_BUILT = -9     /* operation needs a GPU */
+};
+
+/* structure-of-arrays field order of a particle state (src/electron.jl:13-26) */
+enum pt_field_index {
+    PT_X = 0, PT_Y = 1, PT_Z = 2, PT_PX = 3, PT_PY = 4, PT_PZ = 5,
+    PT_W = 6, PT_T = 7, PT_S = 8, PT_R = 9, PT_NFIELDS = 10
+};
+
+typedef struct pt_ctx pt_ctx;
+
+/* One process of a collision table, in table order (the `proc` tuple of
+ * ChebyshevCollisionTable, src/collision_table.jl:290-302). */
+typedef struct {
+    int32_t kind;      /* pt_proc_kind */
+    int32_t Z;         /* target atomic number */
+    int32_t slot;      /* SB or photoelectric data slot (pt_upload_sb / pt_upload_pe) */
+    int32_t N;         /* RBEB occupation number */
+    double B, U;       /* RBEB binding and kinetic energy of the orbital (J) */
+    double tcut;       /* Moller / Bhabha secondary-energy cut (J) */
+    double zf;         /* Z as a real number (annihilation accepts an effective Z) */
+    double c[4];       /* per-process constants computed by the host with libm:
+                          ELASTIC: c[0] = 1.3413 * Z^(-1/3) * a_0 (screening radius);
+                          BETHE_HEITLER: c[0] = Z^(1/3), c[1] = 8 log(Z)/3,
+                          c[2] = c[1] + 8 fc, c[3] unused (src/bethe_heitler.jl:197-204) */
+} pt_proc_desc;
+
+/* ChebyshevCollisionTable (src/collision_table.jl:290-302) of one species */
+typedef struct {
+    int32_t nprocs;      /* L */
+    int32_t order;       /* Chebyshev order, must be 3 */
+    int32_t nintervals;  /* k (intervals 0..k), must be <= 63 */
+    int32_t nswarm;      /* rate bounds given per swarm (>= 1) */
+    double xmax;         /* BinaryIntervals root (J) */
+    double energy_cut;   /* Population energy_cut (J) */
+    const double* rate;      /* [(k+1)][L][order], the reference's column-major (order, L, k+1) */
+    const double* ratebound; /* [nswarm][(k+1)][order] */
+    const pt_proc_desc* procs;
+} pt_table_desc;
+
+typedef struct {
+    int32_t kind;        /* pt_field_kind */
+    int32_t pad;
+    double v[3];         /* HOMOGENEOUS, DOUBLELAYER value; STEP value below z */
+    double v2[3];        /* STEP value above z */
+    double z1, z2;       /* DOUBLELAYER bounds; STEP: z = z1 */
+    double sx, sy, sz, ez0; /* CONFINED */
+} pt_field_fn;
+
+/* ElectromagneticField(e, b), src/field.jl:54-70 */
+typedef struct {
+    pt_field_fn e, b;
+} pt_field_desc;
+
+typedef struct {
+    uint64_t capacity[PT_NSPECIES]; /* slots per species (Population max_particles) */
+    uint64_t seed;                  /* Philox key */
+    int32_t tracked[PT_NSPECIES];   /* 0: particles of this species are tallied at birth,
+                                       not stored */
+    int32_t nswarm;                 /* independent swarms (fields, rate bounds) */
+    int32_t device;                 /* HIP device ordinal */
+    int32_t tally_bins;             /* log-spaced energy bins of the birth tallies */
+    double tally_lo, tally_hi;      /* tally energy range (J) */
+} pt_config;
+
+/* population statistics (src/population.jl:89-223) */
+typedef struct {
+    uint64_t n;              /* slots in use (nparticles) */
+    uint64_t nactive;        /* nactives */
+    double weight;           /* weight */
+    double mean_energy;      /* meanenergy (J) */
+    double centroid[3];      /* spread()[1] */
+    double spread;           /* spread()[2] */
+    double max_energy;       /* maxenergy over active particles */
+} pt_pop_stats;
+
+/* counters of the algorithmic work done (for throughput and roofline figures) */
+typedef struct {
+    uint64_t particle_steps;     /* active particles at step start, summed over steps */
+    uint64_t secondaries;        /* particles appended to a population */
+    uint64_t moved;              /* particles moved by compaction */
+    uint64_t events;             /* real + null collision events */
+    uint64_t steps;
+    uint64_t tail_particles;     /* secondaries advanced within their birth step */
+    double advance_ms;           /* summed duration of the main advance kernel */
+    uint64_t advance_launches;
+    double tail_ms;
+    double compact_ms;
+    uint64_t main_secondaries;   /* particles appended by the main advance kernel */
+} pt_work;
+
+/* ---- lifecycle ---------------------------------------------------------------- */
+int pt_abi_version(void);
+/* Population(...) constructors + MultiPopulation (src/population.jl:34-44,
+ * src/mixed_population.jl:4-12): allocates capacity[s] slots per species. */
+int pt_create(const pt_config* cfg, pt_ctx** out);
+void pt_destroy(pt_ctx* ctx);
+const char* pt_last_error(const pt_ctx* ctx);
+
+/* ---- tables and fields ------------------------------------------------------------ */
+/* Upload a ChebyshevCollisionTable built by collision_table_from_processes
+ * (src/collision_table.jl:115-149) for one species. */
+int pt_upload_table(pt_ctx* ctx, int species, const pt_table_desc* table);
+/* SeltzerBerger sampling data (src/seltzer.jl:9-49): log_energy[ne],
+ * data[ne][ncum] = log(k/T) at pcum = LinRange(0,1,ncum) for each primary energy. */
+int pt_upload_sb(pt_ctx* ctx, int slot, int Z, int ne, int ncum, const double* log_energy,
+                 const double* data);
+/* PhotoElectric binding energies (src/photo_electric.jl:17, :60-76), K shell first. */
+int pt_upload_pe(pt_ctx* ctx, int slot, int Z, int nshell, const double* binding);
+/* pusher forcing ElectromagneticField (src/field.jl:54-70) of one swarm */
+int pt_set_field(pt_ctx* ctx, int swarm, const pt_field_desc* field);
+
+/* ---- particles --------------------------------------------------------------------- */
+/* Append particles (the initial states given to Population(), src/population.jl:34-44).
+ * soa[f] points to n values of field f (pt_field_index); uid and swarm may be NULL
+ * (uids then continue from the context's counter, swarm 0). */
+int pt_add_particles(pt_ctx* ctx, int species, const double* const soa[PT_NFIELDS],
+                     const uint64_t* uid, const int32_t* swarm, size_t n);
+/* Generate n particles on the device: kinetic energy e1 (or log-uniform in [e1, e2]),
+ * isotropic direction (randsphere, src/util.jl:4-12), x = 0, w = 1, t = t0,
+ * s = nextcoll(); uid = uid0 + i. */
+int pt_init_particles(pt_ctx* ctx, int species, size_t n, double e1, double e2,
+                      int loguniform, int swarm, uint64_t uid0, double t0);
+/* Copy a population back (eachparticle, src/population.jl:81).  soa entries may be NULL. */
+int pt_download(pt_ctx* ctx, int species, double* const soa[PT_NFIELDS], uint8_t* active,
+                uint64_t* uid, int32_t* swarm, size_t cap, size_t* n);
+/* Replace the state of a population with n particles (used for checkpoint restore). */
+int pt_set_particles(pt_ctx* ctx, int species, const double* const soa[PT_NFIELDS],
+                     const uint8_t* active, const uint64_t* uid, const int32_t* swarm,
+                     size_t n);
+int pt_count(pt_ctx* ctx, int species, uint64_t* n);
+
+/* ---- the step ------------------------------------------------------------------------ */
+/* init! (src/mixed_population.jl:20-35): r = ratebound(E) of every active particle. */
+int pt_init(pt_ctx* ctx);
+/* One time step: advance! (src/mixed_population.jl:38-47: advance_init!, advance1!
+ * passes, do_one_collision!, collide, apply!, setr!) followed by the deactivation half of
+ * droplow! (src/population.jl:273-284).  Slots are freed by pt_compact.  The step index
+ * of the random streams advances by one. */
+int pt_advance(pt_ctx* ctx, double tfinal);
+/* `nsteps` calls of pt_advance at t0 + dt, t0 + 2dt, ..., with pt_compact every
+ * `compact_every` steps (0: never) and error checks every `check_every` steps. */
+int pt_run(pt_ctx* ctx, double t0, double dt, int nsteps, int compact_every, int check_every);
+/* repack! (src/population.jl:229-259): fill holes below the active count with active
+ * particles from the end (not order preserving, as in the reference). */
+int pt_compact(pt_ctx* ctx, int species);
+/* population statistics (src/population.jl:89-223) over active particles */
+int pt_stats(pt_ctx* ctx, int species, pt_pop_stats* out);
+/* CollisionCounter (src/callback.jl:118-141): out[0] null events, out[1+j] process j in
+ * table order; nmax entries at most.  reset != 0 clears the counters afterwards. */
+int pt_collision_counts(pt_ctx* ctx, int species, int64_t* out, int nmax, int reset);
+/* birth tallies of an untracked species: weight histogram over tally bins, count, weight */
+int pt_tally_read(pt_ctx* ctx, int species, double* hist, int nbins, double* count,
+                  double* weight);
+/* work counters and (if profiling is on) kernel durations measured with HIP events on
+ * the context's stream */
+int pt_work_read(pt_ctx* ctx, pt_work* out, int reset);
+int pt_profile_enable(pt_ctx* ctx, int on);
+/* step index of the random streams (Philox counter word 2) */
+int pt_set_step(pt_ctx* ctx, uint32_t step);
+int pt_get_step(pt_ctx* ctx, uint32_t* step);
+/* error flags accumulated on the device: flags[0] rate-bound violations, [1] particles
+ * lost to a full population, [2] energies above the table range, [3] sampler domain
+ * failures, [4] secondary passes left unfinished.  flags may be NULL (5 entries).
+ * Returns the status of the first nonzero flag. */
+int pt_check(pt_ctx* ctx, int64_t* flags);
+int pt_synchronize(pt_ctx* ctx);
+
+/* ---- test hooks: same device code as the step, on explicit inputs ------------------ */
+/* Philox4x32-10 on the device: out[4i..4i+3] for counters ctr[4i..] and key. */
+int pt_debug_philox(pt_ctx* ctx, const uint32_t* ctr, const uint32_t key[2], uint32_t* out,
+                    size_t n);
+/* the uniforms a particle stream (uid, step) yields: out[0..n) */
+int pt_debug_uniforms(pt_ctx* ctx, uint64_t uid, uint32_t step, double* out, size_t n);
+/* selection of do_one_collision! (src/collisions.jl:153-186) for n particles of a species:
+ * interval index, selected process (-1 null) and the rate bound ratebound(E). */
+int pt_debug_select(pt_ctx* ctx, int species, size_t n, const double* energy,
+                    const double* u, const int32_t* swarm, int32_t* interval,
+                    int32_t* proc, double* rbound);
+/* one RK2 push (src/pusher.jl:41-63) of n states (soa in/out) by dt[i] in swarm 0 field */
+int pt_debug_push(pt_ctx* ctx, int species, size_t n, double* const soa[PT_NFIELDS],
+                  const double* dt);
+/* collide() of process `proc` (table order) for n particles with injected uniforms
+ * u[i*stride ..]: out_kind[i] outcome kind, out_draws[i] uniforms consumed,
+ * out_states[(i*3+q)*PT_NFIELDS + f] the up to 3 resulting states (primary, new, new),
+ * out_species[i*3+q], out_uid[i*3+q]. */
+int pt_debug_collide(pt_ctx* ctx, int species, int proc, size_t n,
+                     const double* const soa[PT_NFIELDS], const uint64_t* uid,
+                     const double* u, int stride, int32_t* out_kind, int32_t* out_draws,
+                     double* out_states, int32_t* out_species, uint64_t* out_uid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

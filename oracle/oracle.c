@@ -1,0 +1,1555 @@
+/*
+ * oracle.c -- CPU restatement of Particulator.jl's null-collision Monte Carlo step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Build with -O2 -ffp-contract=off so that no
+ * multiply-add is fused: Julia does not contract, and the HIP kernels that this code checks
+ * are built the same way on their bit-exact paths.
+ *
+ * Reference files (paths relative to the Particulator.jl repository) are cited on each
+ * function.  Evaluation order follows Julia's parse: a*b*c == (a*b)*c, x^2 == x*x,
+ * x^3 == x*x*x, x^4 == pow(x,4) (Julia's pow_body), tuple/range sums are left folds.
+ *
+ * Pins (what fixes each part independently of this file; tests/test_oracle_*.py):
+ *   Philox4x32-10 ............ Random123 known-answer vectors
+ *   precheb / intervals ...... closed form of frexp at interval edges 2^n keV
+ *   chebfit .................. interpolation identity at the Chebyshev nodes
+ *   cross sections ........... textbook limits (Thomson, KN at 1 MeV), integrals of the
+ *                              differential forms (RBEB dσ/dW, Moller, screened Rutherford,
+ *                              Klein-Nishina, Heitler), BH zero below 2mc^2
+ *   samplers ................. KS against numerically integrated textbook pdfs
+ *   SB inverse CDF ........... closed-form inversion of the piecewise-quadratic cumulative
+ *   push ..................... exact p(t) for uniform E; hyperbolic-motion x(t) to O(dt^3)
+ *   kinematics ............... momentum / energy conservation of every outcome
+ *   step ..................... thread-count invariance, weight/count bookkeeping
+ * Parity unpinned (no reference output exists, the reference cannot run here):
+ *   the exact order of the Julia sortperm tie-break, BSplineKit's Newton iterates,
+ *   Julia's LU result in chebfit (last-ulp differences).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---------------------------------------------------------------- constants */
+/* src/constants.jl:6-166 (values), :157-164 (derived) */
+static const double PI = 3.141592653589793;
+static double C, C2, ME, QE, ALPHA, HBAR, EPS0, NA, EV, KILO, CENTI, KB, ATM, NAIR;
+static double MC2, MC, RE, A0, TWO_PI;
+static const double EPS64 = 2.220446049250313e-16; /* eps(Float64) */
+
+__attribute__((constructor)) static void init_consts(void) {
+    C = 299792458.0;
+    ME = 9.10938356e-31;
+    QE = 1.6021766208e-19;
+    ALPHA = 0.0072973525664;
+    HBAR = 1.0545718001391127e-34;
+    EPS0 = 8.854187817620389e-12;
+    NA = 6.022140857e+23;
+    EV = 1.6021766208e-19;
+    KILO = 1000.0;
+    CENTI = 0.01;
+    KB = 1.38064852e-23;
+    ATM = 101325.0;
+    NAIR = 2.6867811e+25;
+    C2 = C * C;
+    MC2 = ME * C2;
+    MC = ME * C;
+    RE = QE * QE / (ME * C2) / (4.0 * PI * EPS0);
+    A0 = HBAR / (ME * C * ALPHA);
+    TWO_PI = 2.0 * PI;
+}
+
+void or_get_consts(or_consts* o) {
+    o->c = C; o->c2 = C2; o->m_e = ME; o->e = QE; o->alpha = ALPHA; o->hbar = HBAR;
+    o->eps0 = EPS0; o->N_A = NA; o->eV = EV; o->kilo = KILO; o->centi = CENTI;
+    o->k_B = KB; o->atm = ATM; o->r_e = RE; o->a_0 = A0; o->mc2 = MC2; o->mc = MC;
+    o->nair = NAIR;
+}
+
+/* Julia max/min propagate NaN */
+static double jmax(double a, double b) {
+    if (isnan(a) || isnan(b)) return NAN;
+    return a > b ? a : b;
+}
+
+/* ---------------------------------------------------------------- RNG */
+/* Philox4x32-10, Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2,
+ * 3", SC'11.  Multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key increments 0x9E3779B9 /
+ * 0xBB67AE85. */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0; c1 = lo1; c2 = hi0 ^ c3 ^ k1; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* 64 random bits -> uniform in (0,1): top 52 bits + 1/2, exact in binary64 */
+double or_u01(uint64_t bits) { return ((double)(bits >> 12) + 0.5) * 0x1p-52; }
+
+void or_rng_philox(or_rng* g, uint64_t seed, uint64_t uid, uint32_t step) {
+    memset(g, 0, sizeof(*g));
+    g->key[0] = (uint32_t)seed; g->key[1] = (uint32_t)(seed >> 32);
+    g->uid = uid; g->step = step; g->mode = 0;
+}
+
+void or_rng_inject(or_rng* g, const double* u, size_t n) {
+    memset(g, 0, sizeof(*g));
+    g->mode = 1; g->inj = u; g->ninj = n; g->pos = 0;
+}
+
+double or_rand(or_rng* g) {
+    if (g->mode == 1) {
+        g->draw++;
+        if (g->pos >= g->ninj) { g->exhausted = 1; return 0.5; }
+        return g->inj[g->pos++];
+    }
+    uint32_t d = g->draw++;
+    uint32_t blk = d >> 1;
+    if (g->cblock != blk + 1) {
+        uint32_t ctr[4] = {(uint32_t)g->uid, (uint32_t)(g->uid >> 32), g->step, blk};
+        uint32_t o[4];
+        or_philox4x32_10(ctr, g->key, o);
+        g->cache[0] = or_u01(((uint64_t)o[1] << 32) | o[0]);
+        g->cache[1] = or_u01(((uint64_t)o[3] << 32) | o[2]);
+        g->cblock = blk + 1;
+    }
+    return g->cache[d & 1];
+}
+
+/* uid of the k-th particle created by the event that ended at draw index `draw` of the
+ * stream (uid, step): Philox of a counter with the top bit set, which no uniform draw uses */
+uint64_t or_child_uid(uint64_t seed, uint64_t uid, uint32_t step, uint32_t draw, int k) {
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t ctr[4] = {(uint32_t)uid, (uint32_t)(uid >> 32), step,
+                       0x80000000u | ((draw & 0x3FFFFFFFu) << 1) | (uint32_t)(k & 1)};
+    uint32_t o[4];
+    or_philox4x32_10(ctr, key, o);
+    return ((uint64_t)o[1] << 32) | o[0];
+}
+
+/* src/util.jl:17 */
+static double nextcoll(or_rng* g) { return -log(or_rand(g)); }
+
+/* ---------------------------------------------------------------- kinematics */
+static double dot3(const double a[3], const double b[3]) {
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+static double norm3(const double a[3]) { return sqrt(dot3(a, a)); }
+
+/* src/electron.jl:53, src/positron.jl:40, src/photon.jl:52 */
+double or_kinenergy(int species, const double p[3]) {
+    double pp = dot3(p, p);
+    if (species == OR_PHOTON) return sqrt(pp) * C;
+    return sqrt(MC2 * MC2 + C2 * pp) - MC2;
+}
+
+/* src/electron.jl:47, src/positron.jl:34, src/photon.jl:46 */
+double or_speed(int species, double eng) {
+    if (species == OR_PHOTON) return C;
+    double q = MC2 / (MC2 + eng);
+    return C * sqrt(1.0 - q * q);
+}
+
+/* src/electron.jl:51, src/positron.jl:38, src/photon.jl:48 */
+double or_momentum_norm_from_kin(int species, double kin) {
+    if (species == OR_PHOTON) return kin / C;
+    double a = kin + MC2;
+    return sqrt(a * a - MC2 * MC2) / C;
+}
+
+/* gamma, velocity: src/electron.jl:54-56; photon src/photon.jl:47 */
+static void velocity(int species, const double p[3], double v[3]) {
+    if (species == OR_PHOTON) {
+        double f = C / norm3(p);
+        v[0] = p[0] * f; v[1] = p[1] * f; v[2] = p[2] * f;
+        return;
+    }
+    double g = sqrt(1.0 + C2 * dot3(p, p) / (MC2 * MC2));
+    double inv = 1.0 / (ME * g);
+    v[0] = p[0] * inv; v[1] = p[1] * inv; v[2] = p[2] * inv;
+}
+
+/* ---------------------------------------------------------------- Chebyshev */
+/* src/cheby.jl:24-36 */
+void or_interval(int k, double xmax, int i, double* l, double* r) {
+    if (i == 0) {
+        double l1, r1;
+        or_interval(k, xmax, 1, &l1, &r1);
+        *l = 0.0; *r = l1;
+        return;
+    }
+    double lo = ldexp(1.0, i - k - 1);
+    double hi = 2.0 * lo;
+    *l = lo * xmax; *r = hi * xmax;
+}
+
+/* src/cheby.jl:127-143 with chebval :57-81 (T2 = 2*xi*T1 - T0) */
+int or_precheb(double x, int k, double xmax, double t[3]) {
+    double x1 = x / xmax;
+    int l;
+    double s;
+    if (isnan(x1)) { s = x1; l = 0; }
+    else s = frexp(x1, &l);
+    int i = (x1 == 0.0) ? 0 : l + k;
+    double xi;
+    if (i > 0) {
+        xi = 4.0 * s - 3.0;
+    } else {
+        xi = ldexp(1.0, k + 1) * x1 - 1.0;
+        i = 0;
+    }
+    t[0] = 1.0; t[1] = xi; t[2] = 2.0 * xi * xi - 1.0;
+    return i;
+}
+
+/* src/collision_table.jl:84-91: sum(ntuple(k -> rate[k,j,i+1]*t[k], Val(3))) */
+double or_rate(const or_table* tb, int j, int i, const double t[3]) {
+    const double* c = tb->rate + (size_t)tb->order * (j + (size_t)tb->nprocs * i);
+    double acc = c[0] * t[0];
+    for (int o = 1; o < tb->order; o++) acc = acc + c[o] * t[o];
+    return acc;
+}
+
+/* src/collision_table.jl:98-106 */
+double or_ratebound(const or_table* tb, int swarm, int i, const double t[3]) {
+    const double* c = tb->ratebound + (size_t)tb->order * ((size_t)(tb->k + 1) * swarm + i);
+    double acc = c[0] * t[0];
+    for (int o = 1; o < tb->order; o++) acc = acc + c[o] * t[o];
+    return acc;
+}
+
+static int clamp_interval(const or_table* tb, int i) {
+    if (i > tb->k) return tb->k;
+    if (i < 0) return 0;
+    return i;
+}
+
+/* selection part of do_one_collision!, src/collisions.jl:153-186.  Returns the selected
+ * process (0-based) or -1 for a null collision; partials[j] = xi before process j. */
+int or_select(const or_table* tb, double eng, double u, double r, double* partials) {
+    double t[3];
+    int i = clamp_interval(tb, or_precheb(eng, tb->k, tb->xmax, t));
+    double xi = u * r;
+    for (int j = 0; j < tb->nprocs; j++) {
+        if (partials) partials[j] = xi;
+        double nu = or_rate(tb, j, i, t);
+        if (nu > xi) return j;
+        xi -= nu;
+    }
+    if (partials) partials[tb->nprocs] = xi;
+    return -1;
+}
+
+/* ---------------------------------------------------------------- cross sections */
+/* src/relativistic_coulomb.jl:58-75 */
+static double elastic_totalcs(double Z, double K) {
+    K += 1e-4 * EV;
+    double a = 1.3413 * pow(Z, -1.0 / 3.0) * A0;
+    double g = 1.0 + K / (ME * C2);
+    double p = sqrt(K * (K + 2.0 * (ME * C2))) / C;
+    double b = p / (g * ME * C);
+    double al = HBAR * HBAR / (4.0 * (p * p) * (a * a));
+    double b2 = b * b;
+    return PI * (RE * RE) * (Z * Z) / (pow(b, 4.0) * (g * g)) *
+           ((1.0 + al * b2) / (al * (1.0 + al)) + b2 * log(al / (1.0 + al)));
+}
+
+/* src/rbeb.jl:127-150 */
+static double rbeb_sigmaI(double T, double B, double U) {
+    double t1 = T / MC2, b1 = B / MC2, u1 = U / MC2;
+    double bt2 = 1.0 - 1.0 / ((1.0 + t1) * (1.0 + t1));
+    double bb2 = 1.0 - 1.0 / ((1.0 + b1) * (1.0 + b1));
+    double bu2 = 1.0 - 1.0 / ((1.0 + u1) * (1.0 + u1));
+    double t = T / B;
+    double pre = 2.0 * (A0 * A0) * PI * pow(ALPHA, 4.0) / (b1 * ((bb2 + bt2) + bu2));
+    double tt1 = (1.0 + t1) * (1.0 + t1);
+    double v = 1.0 - 1.0 / t;
+    v = v + (b1 * b1 * (t - 1.0)) / (2.0 * tt1);
+    v = v - ((1.0 + 2.0 * t1) * log(t)) / ((1.0 + t) * tt1);
+    v = v - ((t * t - 1.0) * ((bt2 - log(bt2)) + log(2.0 * b1 * (1.0 - bt2)))) / (2.0 * (t * t));
+    return pre * v;
+}
+
+/* src/rbeb.jl:87-111 (differential in w = W/B) */
+double or_rbeb_dsdw(double W, double T, double B, double U) {
+    double t1 = T / MC2, b1 = B / MC2, u1 = U / MC2;
+    double bt2 = 1.0 - 1.0 / ((1.0 + t1) * (1.0 + t1));
+    double bb2 = 1.0 - 1.0 / ((1.0 + b1) * (1.0 + b1));
+    double bu2 = 1.0 - 1.0 / ((1.0 + u1) * (1.0 + u1));
+    double t = T / B, w = W / B;
+    double pre = 2.0 * (A0 * A0) * PI * pow(ALPHA, 4.0) / (b1 * ((bb2 + bt2) + bu2));
+    double tw = t - w, w1 = 1.0 + w;
+    double tt1 = (1.0 + t1) * (1.0 + t1);
+    double v = b1 * b1 / tt1 + 1.0 / (tw * tw);
+    v = v + 1.0 / (w1 * w1);
+    v = v - ((1.0 + 2.0 * t1) * (1.0 / tw + 1.0 / w1)) / ((1.0 + t) * tt1);
+    v = v + (1.0 / (tw * tw * tw) + 1.0 / (w1 * w1 * w1)) *
+                ((-bt2 - log(2.0 * b1)) + log(bt2 / (1.0 - bt2)));
+    return pre * v;
+}
+
+/* src/moller.jl:40-57 */
+static double moller_totalcs(double Z, double tcut, double eng) {
+    double x = tcut / eng;
+    if (x > 1.0) return 0.0;
+    double g = 1.0 + eng / MC2;
+    double g2 = g * g;
+    double b2 = (g2 - 1.0) / g2;
+    double A = ((g - 1.0) * (g - 1.0) / g2) * (0.5 - x) + 1.0 / x;
+    A = A - 1.0 / (1.0 - x);
+    A = A - ((2.0 * g - 1.0) / g2) * log((1.0 - x) / x);
+    return jmax(0.0, TWO_PI * (RE * RE) * Z * A / (g - 1.0) / b2);
+}
+
+/* src/bhaba.jl:84-91 */
+static void bhabha_bs(double y, double B[4]) {
+    double q = 1.0 - 2.0 * y;
+    B[0] = 2.0 - y * y;
+    B[1] = q * (3.0 + y * y);
+    B[2] = q * q + q * q * q;
+    B[3] = q * q * q;
+}
+
+/* src/bhaba.jl:35-49 */
+static double bhabha_totalcs(double Z, double tcut, double eng) {
+    double g = 1.0 + eng / MC2;
+    double b = sqrt((g * g - 1.0) / (g * g));
+    double x = tcut / eng;
+    double y = 1.0 / (g + 1.0);
+    double B[4];
+    bhabha_bs(y, B);
+    double A = (1.0 / x - 1.0) / (b * b) + B[0] * log(x);
+    A = A + B[1] * (1.0 - x);
+    A = A - B[2] * (1.0 - x * x) / 2.0;
+    A = A + B[3] * (1.0 - x * x * x) / 3.0;
+    return jmax(0.0, TWO_PI * (RE * RE) * Z * A / (g - 1.0));
+}
+
+/* src/anihilation.jl:25-32 */
+static double annih_totalcs(double Z, double eng) {
+    double g = 1.0 + eng / MC2;
+    double g2 = g * g;
+    double sq = sqrt(g2 - 1.0);
+    double A = ((g2 + 4.0 * g + 1.0) / (g2 - 1.0) * log(g + sq) - (g + 3.0) / sq) / (g + 1.0);
+    return Z * PI * (RE * RE) * A;
+}
+
+/* src/compton.jl:34-47 */
+static double kn_totalcs(double Z, double eng) {
+    double x = 2.0 * eng / MC2;
+    double sigma = TWO_PI * (RE * RE) * (1.0 / x) *
+                   ((((1.0 - 4.0 / x) - 8.0 / (x * x)) * log(1.0 + x) + 0.5) + 8.0 / x -
+                    1.0 / ((1.0 + x) * (1.0 + x)) / 2.0);
+    return Z * sigma;
+}
+
+/* src/compton.jl:59-112 */
+static double compton_totalcs(double Zd, double eng) {
+    double Z = Zd;
+    const double barn = 1e-28;
+    const double a = 20.0, b = 230.0, c = 440.0;
+    double d1 = 2.7965e-1 * barn, d2 = -1.8300e-1 * barn, d3 = 6.7527 * barn,
+           d4 = -1.9798e+1 * barn;
+    double e1 = 1.9756e-5 * barn, e2 = -1.0205e-2 * barn, e3 = -7.3913e-2 * barn,
+           e4 = 2.7079e-2 * barn;
+    double f1 = -3.9178e-7 * barn, f2 = 6.8241e-5 * barn, f3 = 6.0480e-5 * barn,
+           f4 = 3.0274e-4 * barn;
+    double Z2 = Z * Z;
+    double p1Z = Z * ((d1 + e1 * Z) + f1 * Z2);
+    double p2Z = Z * ((d2 + e2 * Z) + f2 * Z2);
+    double p3Z = Z * ((d3 + e3 * Z) + f3 * Z2);
+    double p4Z = Z * ((d4 + e4 * Z) + f4 * Z2);
+    double T0 = 15.0e3 * EV;
+    double X = (eng > T0 ? eng : T0) / MC2;
+    double sigma = p1Z * log(1.0 + 2.0 * X) / X +
+                   ((p2Z + p3Z * X) + p4Z * (X * X)) /
+                       (((1.0 + a * X) + b * (X * X)) + c * (X * X * X));
+    if (eng < T0) {
+        double dT0 = 1e3 * EV;
+        X = (T0 + dT0) / MC2;
+        double sigma1 = p1Z * log(1.0 + 2.0 * X) / X +
+                        ((p2Z + p3Z * X) + p4Z * (X * X)) /
+                            (((1.0 + a * X) + b * (X * X)) + c * (X * X * X));
+        double c1 = -T0 * (sigma1 - sigma) / (sigma * dT0);
+        double c2 = 0.150;
+        if (Z > 1.5) c2 = 0.375 - 0.0556 * log(Z);
+        double y = log(eng / T0);
+        sigma *= exp(-y * (c1 + c2 * y));
+    }
+    return sigma;
+}
+
+/* src/photo_electric.jl:54-58 */
+static double pe_totalcs(const or_pe* pe, double eng) {
+    int j = -1;
+    for (int q = 0; q < pe->nint; q++)
+        if (pe->left_energy[q] <= eng) j = q;
+    if (j < 0) return 0.0;
+    double e1 = 1.0 / eng;
+    double ep[4] = {e1, e1 * e1, pow(eng, -3.0), pow(eng, -4.0)};
+    double acc = ep[0] * pe->coeffs[0 + 4 * j];
+    for (int i = 1; i < 4; i++) acc = acc + ep[i] * pe->coeffs[i + 4 * j];
+    return acc;
+}
+
+/* src/bethe_heitler.jl:28-80 */
+static double bh_totalcs(double Zd, double eng) {
+    double Z = Zd;
+    if (eng < 2.0 * MC2) return 0.0;
+    const double mb = 1e-34;
+    double lim = 1.5e6 * EV;
+    double a0 = 8.7842e+2 * mb, a1 = -1.9625e+3 * mb, a2 = 1.2949e+3 * mb,
+           a3 = -2.0028e+2 * mb, a4 = 1.2575e+1 * mb, a5 = -2.8333e-1 * mb;
+    double b0 = -1.0342e+1 * mb, b1 = 1.7692e+1 * mb, b2 = -8.2381 * mb, b3 = 1.3063 * mb,
+           b4 = -9.0815e-2 * mb, b5 = 2.3586e-3 * mb;
+    double c0 = -4.5263e+2 * mb, c1 = 1.1161e+3 * mb, c2 = -8.6749e+2 * mb,
+           c3 = 2.1773e+2 * mb, c4 = -2.0467e+1 * mb, c5 = 6.5372e-1 * mb;
+    double eng1 = eng > lim ? eng : lim;
+    double x = log(eng1 / MC2);
+    double x2 = x * x, x3 = x2 * x, x4 = x3 * x, x5 = x4 * x;
+    double F1 = ((((a0 + a1 * x) + a2 * x2) + a3 * x3) + a4 * x4) + a5 * x5;
+    double F2 = ((((b0 + b1 * x) + b2 * x2) + b3 * x3) + b4 * x4) + b5 * x5;
+    double F3 = ((((c0 + c1 * x) + c2 * x2) + c3 * x3) + c4 * x4) + c5 * x5;
+    double sigma = (Z + 1.0) * ((F1 * Z + F2 * Z * Z) + F3);
+    if (eng < lim) {
+        double q = (eng - 2.0 * MC2) / (lim - 2.0 * MC2);
+        sigma = sigma * (q * q);
+    }
+    return sigma;
+}
+
+/* src/seltzer.jl:217-231 */
+static double sb_totalcs(const or_sb* sb, double K) {
+    double lk = log(K);
+    int i = -1;
+    for (int q = 0; q < sb->ne; q++)
+        if (sb->log_energy[q] <= lk) i = q;
+    if (i < 0) return 0.0;
+    if (i + 1 >= sb->ne) return NAN; /* reference: out-of-bounds read */
+    double w = (sb->log_energy[i + 1] - lk) / (sb->log_energy[i + 1] - sb->log_energy[i]);
+    return w * sb->totalcs[i] + (1.0 - w) * sb->totalcs[i + 1];
+}
+
+double or_totalcs(const or_proc* p, double eng) {
+    double Z = p->zf != 0.0 ? p->zf : (double)p->Z;
+    switch (p->kind) {
+    case OR_ELASTIC: return elastic_totalcs(Z, eng);
+    case OR_SB: return sb_totalcs(p->sb, eng);
+    case OR_RBEB: return eng > p->B ? (double)p->N * rbeb_sigmaI(eng, p->B, p->U) : 0.0;
+    case OR_MOLLER: return moller_totalcs(Z, p->tcut, eng);
+    case OR_COMPTON: return compton_totalcs(Z, eng);
+    case OR_KN_COMPTON: return kn_totalcs(Z, eng);
+    case OR_PHOTO: return pe_totalcs(p->pe, eng);
+    case OR_BH: return bh_totalcs(Z, eng);
+    case OR_BHABHA: return bhabha_totalcs(Z, p->tcut, eng);
+    case OR_ANNIH: return annih_totalcs(Z, eng);
+    default: return 0.0;
+    }
+}
+
+/* ---------------------------------------------------------------- table builder */
+/* 3x3 (n x n) solve by LU with partial pivoting, standing in for Julia's A \ fx
+ * (src/cheby.jl:222) */
+static void lu_solve(int n, double* A, double* b) {
+    for (int c = 0; c < n; c++) {
+        int pr = c;
+        double best = fabs(A[c * n + c]);
+        for (int r = c + 1; r < n; r++)
+            if (fabs(A[r * n + c]) > best) { best = fabs(A[r * n + c]); pr = r; }
+        if (pr != c) {
+            for (int q = 0; q < n; q++) {
+                double tmp = A[c * n + q]; A[c * n + q] = A[pr * n + q]; A[pr * n + q] = tmp;
+            }
+            double tb = b[c]; b[c] = b[pr]; b[pr] = tb;
+        }
+        for (int r = c + 1; r < n; r++) {
+            double f = A[r * n + c] / A[c * n + c];
+            A[r * n + c] = f;
+            for (int q = c + 1; q < n; q++) A[r * n + q] -= f * A[c * n + q];
+            b[r] -= f * b[c];
+        }
+    }
+    for (int r = n - 1; r >= 0; r--) {
+        double s = b[r];
+        for (int q = r + 1; q < n; q++) s -= A[r * n + q] * b[q];
+        b[r] = s / A[r * n + r];
+    }
+}
+
+typedef double (*or_fn)(void* ctx, double x);
+
+/* src/cheby.jl:211-227, nodes :41-43 */
+static void chebfit_fn(or_fn f, void* ctx, int k, double xmax, int n, double* out) {
+    for (int i = 0; i <= k; i++) {
+        double l, r;
+        or_interval(k, xmax, i, &l, &r);
+        double A[64], fx[8];
+        for (int m = 0; m < n; m++) {
+            double x = (l + r) / 2.0 + (r - l) / 2.0 * cos((double)(2 * m + 1) * PI / (double)(2 * n));
+            fx[m] = f(ctx, x);
+            double xi = (2.0 * x - (l + r)) / (r - l);
+            double t0 = 1.0, t1 = xi;
+            for (int q = 0; q < n; q++) {
+                double tq;
+                if (q == 0) tq = 1.0;
+                else if (q == 1) tq = xi;
+                else { tq = 2.0 * xi * t1 - t0; t0 = t1; t1 = tq; }
+                A[m * n + q] = tq;
+            }
+        }
+        lu_solve(n, A, fx);
+        for (int q = 0; q < n; q++) out[q + (size_t)n * i] = fx[q];
+    }
+}
+
+typedef struct { const or_proc* p; int species; } proc_ctx;
+static double proc_rate_fn(void* c, double eng) {
+    proc_ctx* pc = (proc_ctx*)c;
+    return pc->p->coef * or_speed(pc->species, eng) * or_totalcs(pc->p, eng);
+}
+
+int or_chebfit_proc(const or_proc* p, int species, int k, double xmax, int order, double* out) {
+    proc_ctx pc = {p, species};
+    chebfit_fn(proc_rate_fn, &pc, k, xmax, order, out);
+    return 0;
+}
+
+/* chebeval / chebdiff over a[:, j, :] (src/cheby.jl:111-119, :180-203) */
+static double cheb_eval_j(const double* a, int nprocs, int j, int k, double xmax, int order,
+                          double x) {
+    double t[3];
+    int i = or_precheb(x, k, xmax, t);
+    if (i > k) return NAN;
+    const double* c = a + (size_t)order * (j + (size_t)nprocs * i);
+    double acc = c[0] * t[0];
+    for (int o = 1; o < order; o++) acc = acc + c[o] * t[o];
+    return acc;
+}
+
+static double cheb_diff_j(const double* a, int nprocs, int j, int k, double xmax, int order,
+                          double x) {
+    double t[3];
+    int i = or_precheb(x, k, xmax, t);
+    if (i > k) return NAN;
+    double xi = t[1];
+    double l, r;
+    or_interval(k, xmax, i, &l, &r);
+    const double* c = a + (size_t)order * (j + (size_t)nprocs * i);
+    double u0 = 1.0, u1 = 2.0 * xi;
+    double df = c[1];
+    for (int q = 3; q <= order; q++) {
+        df += (double)(q - 1) * u1 * c[q - 1];
+        double un = 2.0 * xi * u1 - u0;
+        u0 = u1; u1 = un;
+    }
+    return 2.0 * df / (r - l);
+}
+
+typedef struct { const double* a; int nprocs, species, k, order; double xmax, Fdt, safety; } rb_ctx;
+/* src/collision_table.jl:152-167 */
+static double rb_fn(void* c, double eng) {
+    rb_ctx* q = (rb_ctx*)c;
+    if (q->nprocs == 0) return 0.0;
+    double df = cheb_diff_j(q->a, q->nprocs, 0, q->k, q->xmax, q->order, eng);
+    for (int j = 1; j < q->nprocs; j++)
+        df = df + cheb_diff_j(q->a, q->nprocs, j, q->k, q->xmax, q->order, eng);
+    double f = cheb_eval_j(q->a, q->nprocs, 0, q->k, q->xmax, q->order, eng);
+    for (int j = 1; j < q->nprocs; j++)
+        f = f + cheb_eval_j(q->a, q->nprocs, j, q->k, q->xmax, q->order, eng);
+    double g = (q->species == OR_PHOTON) ? 0.0 : 1.0 + eng / MC2;
+    return f + g * g * g * or_speed(q->species, eng) * q->Fdt * fabs(df) * q->safety;
+}
+
+int or_compratebound(const double* a, int nprocs, int species, double Fdt, double safety,
+                     int k, double xmax, int order, double* rb_out) {
+    rb_ctx q = {a, nprocs, species, k, order, xmax, Fdt, safety};
+    chebfit_fn(rb_fn, &q, k, xmax, order, rb_out);
+    return 0;
+}
+
+/* Julia isless for the reverse-ordered stable sortperm (NaN keys sort first) */
+static int key_before(double a, double b) { /* a strictly before b in rev order */
+    if (isnan(a)) return !isnan(b) ? 1 : 0;
+    if (isnan(b)) return 0;
+    return a > b;
+}
+
+/* src/collision_table.jl:115-149.  rate_out (order, nprocs, k+1) in sorted order;
+ * perm_out[j] = original index of sorted process j. */
+int or_build_table(const or_proc* procs, int nprocs, int species, double Fdt, double safety,
+                   int k, double xmax, int order, double* rate_out, double* rb_out,
+                   int* perm_out, double* sortkey_out) {
+    size_t per = (size_t)order * (k + 1);
+    double* a = (double*)malloc(sizeof(double) * per * (nprocs > 0 ? nprocs : 1));
+    double* a1 = (double*)malloc(sizeof(double) * per);
+    for (int j = 0; j < nprocs; j++) {
+        or_chebfit_proc(&procs[j], species, k, xmax, order, a1);
+        for (int i = 0; i <= k; i++)
+            for (int o = 0; o < order; o++)
+                a[o + (size_t)order * (j + (size_t)nprocs * i)] = a1[o + (size_t)order * i];
+    }
+    or_compratebound(a, nprocs, species, Fdt, safety, k, xmax, order, rb_out);
+    /* LogLinRange(log(1e-2 eV), log(0.9999 max_energy), 100_000), src/util.jl:60-77 */
+    const int NE = 100000;
+    double L1 = log(1e-2 * EV), L2 = log(0.9999 * xmax);
+    double x0 = exp(L1);
+    double* keys = (double*)malloc(sizeof(double) * (nprocs > 0 ? nprocs : 1));
+    for (int j = 0; j < nprocs; j++) keys[j] = -INFINITY;
+    for (int m = 0; m < NE; m++) {
+        double t = (double)m / (double)(NE - 1);
+        double L = (1.0 - t) * L1 + t * L2;
+        double eng = exp(L) - x0;
+        double v1 = or_speed(species, eng);
+        for (int j = 0; j < nprocs; j++) {
+            double val = or_totalcs(&procs[j], eng) * v1;
+            if (isnan(val) || isnan(keys[j])) keys[j] = NAN;
+            else if (val > keys[j]) keys[j] = val;
+        }
+    }
+    for (int j = 0; j < nprocs; j++) keys[j] = procs[j].coef * keys[j];
+    int* perm = perm_out;
+    for (int j = 0; j < nprocs; j++) perm[j] = j;
+    /* stable insertion sort, descending, NaN first */
+    for (int a2 = 1; a2 < nprocs; a2++) {
+        int v = perm[a2], q = a2 - 1;
+        while (q >= 0 && key_before(keys[v], keys[perm[q]])) { perm[q + 1] = perm[q]; q--; }
+        perm[q + 1] = v;
+    }
+    for (int j = 0; j < nprocs; j++) {
+        if (sortkey_out) sortkey_out[j] = keys[j];
+        for (int i = 0; i <= k; i++)
+            for (int o = 0; o < order; o++)
+                rate_out[o + (size_t)order * (j + (size_t)nprocs * i)] =
+                    a[o + (size_t)order * (perm[j] + (size_t)nprocs * i)];
+    }
+    free(keys); free(a); free(a1);
+    return 0;
+}
+
+/* ---------------------------------------------------------------- SB table build */
+/* cumulative of the piecewise-linear interpolant of p over knots x (the order-2 B-spline
+ * of src/seltzer.jl:164 and its integral :165) */
+static double pl_cum(const double* x, const double* p, int n, double xq) {
+    if (xq <= x[0]) return p[0] * (xq - x[0]);
+    double acc = 0.0;
+    for (int m = 0; m < n - 1; m++) {
+        double h = x[m + 1] - x[m];
+        if (xq <= x[m + 1]) {
+            double d = xq - x[m];
+            double slope = (p[m + 1] - p[m]) / h;
+            return acc + p[m] * d + slope * d * d / 2.0;
+        }
+        acc += (p[m] + p[m + 1]) * h / 2.0;
+    }
+    return acc + p[n - 1] * (xq - x[n - 1]);
+}
+static double pl_val(const double* x, const double* p, int n, double xq) {
+    if (xq <= x[0]) return p[0];
+    for (int m = 0; m < n - 1; m++)
+        if (xq < x[m + 1]) return p[m] + (p[m + 1] - p[m]) * (xq - x[m]) / (x[m + 1] - x[m]);
+    return p[n - 1];
+}
+
+/* src/seltzer.jl:163-192: Newton inversion of the normalised cumulative, rtol 1e-6 */
+int or_sb_findcum(const double* x, const double* p, int n, const double* pcum, int npcum,
+                  double xmin, double xmax, double* out) {
+    double lo = xmin > x[0] ? xmin : x[0];
+    double hi = xmax < x[n - 1] ? xmax : x[n - 1];
+    double cum0 = pl_cum(x, p, n, lo), cum1 = pl_cum(x, p, n, hi);
+    /* knots of the integral spline: x with the end knots repeated 3 times */
+    int nk = n + 4;
+    double* knt = (double*)malloc(sizeof(double) * nk);
+    double* fk = (double*)malloc(sizeof(double) * nk);
+    knt[0] = knt[1] = knt[2] = x[0];
+    for (int m = 1; m < n - 1; m++) knt[2 + m] = x[m];
+    knt[n + 1] = knt[n + 2] = knt[n + 3] = x[n - 1];
+    nk = n + 4;
+    for (int m = 0; m < nk; m++) fk[m] = (pl_cum(x, p, n, knt[m]) - cum0) / (cum1 - cum0);
+    for (int q = 0; q < npcum; q++) {
+        int j = -1;
+        for (int m = 0; m < nk; m++)
+            if (fk[m] <= pcum[q]) j = m;
+        int ja = j < 0 ? 0 : j;
+        int jb = (j + 1 > nk - 1) ? nk - 1 : j + 1;
+        double xs = 0.5 * (knt[ja] + knt[jb]);
+        double dx = INFINITY;
+        int it = 0;
+        while (fabs(dx / xs) > 1e-6 && it < 200) {
+            double f = (pl_cum(x, p, n, xs) - cum0) / (cum1 - cum0);
+            double df = pl_val(x, p, n, xs) / (cum1 - cum0);
+            dx = (f - pcum[q]) / df;
+            xs = xs - dx;
+            it++;
+        }
+        out[q] = xs;
+    }
+    free(knt); free(fk);
+    return 0;
+}
+
+/* src/seltzer.jl:23-49 (gamma_min 100 eV, gamma_max 50 MeV, energy scale 1 MeV) */
+int or_sb_build(int Z, const double* kfrac, int nx, const double* y, int ny,
+                const double* values, int ncum, double* log_energy, double* totalcs,
+                double* data) {
+    double gmin = 1e2 * EV, gmax = 5e7 * EV, escale = 1e6 * EV;
+    double* logk = (double*)malloc(sizeof(double) * nx);
+    double* pc = (double*)malloc(sizeof(double) * ncum);
+    for (int q = 0; q < nx; q++) logk[q] = log(kfrac[q]);
+    for (int q = 0; q < ncum; q++) pc[q] = (double)q / (double)(ncum - 1);
+    for (int i = 0; i < ny; i++) log_energy[i] = log(exp(y[i]) * escale);
+    for (int i = 0; i < ny; i++) {
+        double T1 = exp(log_energy[i]);
+        double r = 1.0 + T1 / MC2;
+        double beta = sqrt(1.0 - 1.0 / (r * r));
+        const double* row = values + (size_t)nx * i;
+        double xmin = log(gmin / T1), xmax = log(gmax / T1);
+        or_sb_findcum(logk, row, nx, pc, ncum, xmin, xmax, data + (size_t)ncum * i);
+        double lo = xmin > logk[0] ? xmin : logk[0];
+        double hi = xmax < logk[nx - 1] ? xmax : logk[nx - 1];
+        double cs = pl_cum(logk, row, nx, hi) - pl_cum(logk, row, nx, lo);
+        totalcs[i] = ((double)(Z * Z) / (beta * beta)) * 1e-31 * cs;
+    }
+    free(logk); free(pc);
+    return 0;
+}
+
+/* ---------------------------------------------------------------- photoelectric data */
+/* Sandia rows for N, O, Ar (src/static_sandia_data.jl:134-143, :145-156, :254-265),
+ * Z/A ratios (:1405-1441) and binding energies (src/atomic_shells.jl:37, :40, :70). */
+static const double SANDIA_N[8][5] = {
+    {0.01, 0.1010E+05, 0.0000E+00, 0.0000E+00, 0.0000E+00},
+    {0.0404, -0.3622E+03, 0.3873E+03, 0.1244E+02, -0.4452E+00},
+    {0.4016, -0.2338E+04, 0.5732E+04, -0.2082E+03, 0.1482E+03},
+    {0.8, -0.4940E+01, -0.8442E+02, 0.4620E+04, -0.1186E+04},
+    {4.0, 0.2019E+01, -0.1249E+03, 0.4609E+04, -0.9421E+03},
+    {20.0, 0.1709E-01, -0.8196E+01, 0.2345E+04, 0.1369E+05},
+    {100.0, 0.1872E-02, -0.6732E+00, 0.1282E+04, 0.5700E+05},
+    {500.0, 0.8122E-03, 0.8364E+00, 0.4410E+03, 0.2358E+06}};
+static const double SANDIA_O[8][5] = {
+    {0.01, 9.343E+03, 9.026E+02, -2.467E+01, 1.505E-01},
+    {0.02, 6.034E+03, 7.319E+02, -2.677E+01, 2.842E-01},
+    {0.0483, -0.2863E+03, 0.4085E+03, 0.4436E+02, -0.1782E+01},
+    {0.532, -0.7181E+02, 0.4748E+03, 0.5542E+04, -0.1363E+04},
+    {4.0, 0.2745E+01, -0.1747E+03, 0.7159E+04, -0.2213E+04},
+    {20.0, 0.3774E-01, -0.1559E+02, 0.4045E+04, 0.1810E+05},
+    {100.0, 0.3169E-02, -0.1146E+01, 0.2194E+04, 0.9131E+05},
+    {500.0, 0.1367E-02, 0.1473E+01, 0.7214E+03, 0.4048E+06}};
+static const double SANDIA_AR[7][5] = {
+    {0.01, -8.294E+04, 5.720E+03, -1.101E+02, 6.668E-01},
+    {0.04, -1.401E+03, 9.577E+02, -7.842E+01, 1.797E+00},
+    {0.245, -0.2244E+03, 0.1388E+04, 0.2571E+04, -0.5113E+03},
+    {3.203, -0.1446E+02, 0.4359E+03, 0.6578E+05, -0.7284E+05},
+    {20.0, 0.8786E+00, -0.3001E+03, 0.7682E+05, -0.1219E+06},
+    {100.0, 0.6779E-01, -0.2695E+02, 0.5069E+05, 0.5281E+06},
+    {500.0, 0.2598E-01, 0.3454E+02, 0.1059E+05, 0.1068E+08}};
+static const double SHELLS_N[4] = {403.0, 37.3, 20.33, 14.53};
+static const double SHELLS_O[4] = {543.1, 41.6, 28.48, 13.62};
+static const double SHELLS_AR[7] = {3206.3, 326.5, 250.6, 248.5, 29.24, 15.94, 15.76};
+
+/* src/photo_electric.jl:19-32 */
+int or_pe_build(int Z, double* left_energy, double* coeffs, double* binding, int* nint,
+                int* nshell) {
+    const double(*rows)[5];
+    const double* sh;
+    double zoa;
+    int ni, ns;
+    if (Z == 7) { rows = SANDIA_N; ni = 8; sh = SHELLS_N; ns = 4; zoa = 0.4998; }
+    else if (Z == 8) { rows = SANDIA_O; ni = 8; sh = SHELLS_O; ns = 4; zoa = 0.5; }
+    else if (Z == 18) { rows = SANDIA_AR; ni = 7; sh = SHELLS_AR; ns = 7; zoa = 0.4506; }
+    else return -1;
+    double A = (double)Z / zoa;
+    double kev = KILO * EV;
+    double s[4] = {kev, pow(kev, 2.0), pow(kev, 3.0), pow(kev, 4.0)};
+    double conv = CENTI * CENTI * A / NA;
+    for (int j = 0; j < ni; j++) {
+        left_energy[j] = rows[j][0] * kev;
+        for (int i = 0; i < 4; i++) coeffs[i + 4 * j] = rows[j][i + 1] * s[i] * conv;
+    }
+    for (int q = 0; q < ns; q++) binding[q] = sh[q] * EV;
+    *nint = ni; *nshell = ns;
+    return 0;
+}
+
+/* ---------------------------------------------------------------- samplers */
+/* src/util.jl:4-12 */
+void or_randsphere(or_rng* g, double out[3]) {
+    double phi = TWO_PI * or_rand(g);
+    double sp = sin(phi), cp = cos(phi);
+    double u = 2.0 * or_rand(g) - 1.0;
+    double v = sqrt(1.0 - u * u);
+    out[0] = v * cp; out[1] = v * sp; out[2] = u;
+}
+
+/* src/util.jl:40-57 (`turn`).  Deviations where the reference returns NaN: 1 - cos^2 < 0
+ * from rounding is clamped to 0, and a direction exactly along +-z (s == 0) uses the
+ * axis-aligned rotation. */
+void or_turn(const double u[3], double cost, double phi, double n, double out[3]) {
+    double nrm = norm3(u);
+    double m0 = u[0] / nrm, m1 = u[1] / nrm, m2 = u[2] / nrm;
+    double sp = sin(phi), cp = cos(phi);
+    double q = 1.0 - cost * cost;
+    double sint = sqrt(q > 0.0 ? q : 0.0);
+    double qs = 1.0 - m2 * m2;
+    double s = sqrt(qs > 0.0 ? qs : 0.0);
+    double x, y, z;
+    if (s > 0.0) {
+        double bx = m0 * m2 * cp + (-m1) * sp;
+        double by = m1 * m2 * cp + m0 * sp;
+        x = sint * bx / s + m0 * cost;
+        y = sint * by / s + m1 * cost;
+        z = -s * sint * cp + m2 * cost;
+    } else {
+        x = sint * cp;
+        y = sint * sp;
+        z = m2 * cost;
+    }
+    out[0] = n * x; out[1] = n * y; out[2] = n * z;
+}
+
+/* src/util.jl:143-159 */
+double or_sample_tsai(double T, or_rng* g) {
+    double umax = 2.0 * (1.0 + T / (ME * C2));
+    const double a1 = 1.6, a2 = 1.6 / 3.0, border = 0.25;
+    double u;
+    for (;;) {
+        double r1 = or_rand(g);
+        double r2 = or_rand(g);
+        double uu = -log(r1 * r2);
+        u = border > or_rand(g) ? uu * a1 : uu * a2;
+        if (u <= umax) break;
+        if (g->exhausted) break;
+    }
+    return 1.0 - 2.0 * (u * u) / (umax * umax);
+}
+
+/* src/relativistic_coulomb.jl:35-51 */
+double or_sample_rel_sr(double alpha, double beta, or_rng* g) {
+    double x;
+    for (;;) {
+        double u = or_rand(g);
+        x = alpha * u / (alpha + 1.0 - u);
+        double z = or_rand(g);
+        if (z < 1.0 - beta * beta * x) break;
+        if (g->exhausted) break;
+    }
+    return 1.0 - 2.0 * x;
+}
+
+/* src/rbeb.jl:156-196 */
+double or_rbeb_sample(double T, double B, or_rng* g) {
+    double t1 = T / (ME * C2);
+    double b1 = B / (ME * C2);
+    double bt2 = 1.0 - 1.0 / ((1.0 + t1) * (1.0 + t1));
+    double t = T / B;
+    double A = -(1.0 + 2.0 * t1) / (t + 1.0) / ((1.0 + t1) * (1.0 + t1));
+    const double Bc = 1.0;
+    double Cc = (log(bt2 / (1.0 - bt2)) - bt2) - log(2.0 * b1);
+    double M = b1 * b1 / ((1.0 + t1) * (1.0 + t1));
+    double w;
+    for (;;) {
+        double u = or_rand(g);
+        w = u / ((t + 1.0) / (t - 1.0) - u);
+        double w1 = 1.0 + w, tw = t - w;
+        double pb = (2.0 * Bc + 2.0 * Cc + (t + 1.0) * (t + 1.0) * Bc * M / 4.0) / (w1 * w1);
+        double wa = w + 1.0;
+        double g1 = 1.0 / wa + 1.0 / tw;
+        double g2 = 1.0 / (wa * wa) + 1.0 / (tw * tw);
+        double g3 = 1.0 / (wa * wa * wa) + 1.0 / (tw * tw * tw);
+        double p0 = A * g1 + Bc * (g2 + M) + Cc * g3;
+        if (or_rand(g) * pb < p0) break;
+        if (g->exhausted) break;
+    }
+    return B * w;
+}
+
+/* src/moller.jl:64-87 */
+double or_moller_sample(double tcut, double eng, or_rng* g) {
+    double e0 = tcut / eng;
+    double gm = 1.0 + eng / MC2;
+    double eps;
+    for (;;) {
+        double r = or_rand(g);
+        eps = e0 / (1.0 - r + 2.0 * e0 * r);
+        double gg = 4.0 / (9.0 * (gm * gm) - 10.0 * gm + 5.0) *
+                    ((gm - 1.0) * (gm - 1.0) * (eps * eps) -
+                     (2.0 * (gm * gm) + 2.0 * gm - 1.0) * (eps / (1.0 - eps)) +
+                     gm * gm / ((1.0 - eps) * (1.0 - eps)));
+        if (or_rand(g) < gg) break;
+        if (g->exhausted) break;
+    }
+    return eps * eng;
+}
+
+/* src/bhaba.jl:55-81 */
+double or_bhabha_sample(double tcut, double eng, or_rng* g) {
+    double e0 = tcut / eng;
+    double gm = 1.0 + eng / MC2;
+    double y = 1.0 / (gm + 1.0);
+    double B0 = gm * gm / (gm * gm - 1.0);
+    double B[4];
+    bhabha_bs(y, B);
+    double eps;
+    for (;;) {
+        double r = or_rand(g);
+        eps = e0 / (1.0 - r + e0 * r);
+        double g1 = (((B0 + B[0] * eps) + B[1] * (eps * eps)) + B[2] * (eps * eps * eps)) +
+                    B[3] * pow(eps, 4.0);
+        double g2 = (((B0 + B[0] * e0) + B[1] * (e0 * e0)) + B[2] * (e0 * e0 * e0)) +
+                    B[3] * pow(e0, 4.0);
+        double gg = g1 / g2;
+        if (or_rand(g) < gg) break;
+        if (g->exhausted) break;
+    }
+    return eps * eng;
+}
+
+/* Julia searchsortedfirst on LinRange(0,1,ncum): first i with (i-1)/(ncum-1) >= x (1-based) */
+static int pcum_first(double x, int ncum) {
+    double d = (double)(ncum - 1);
+    int i = (int)(x * d) + 1;
+    if (i < 1) i = 1;
+    if (i > ncum) i = ncum;
+    while (i > 1 && (double)(i - 2) / d >= x) i--;
+    while (i < ncum && (double)(i - 1) / d < x) i++;
+    return i;
+}
+
+/* src/seltzer.jl:97-122 */
+double or_sb_sample(const or_sb* sb, double T, or_rng* g) {
+    double x = or_rand(g);
+    double y = log(T);
+    int ncum = sb->ncum;
+    double d = (double)(ncum - 1);
+    int i2 = pcum_first(x, ncum), i1 = i2 - 1;
+    int j2 = 1; /* searchsortedfirst(log_energy, y), 1-based */
+    while (j2 <= sb->ne && sb->log_energy[j2 - 1] < y) j2++;
+    int j1 = j2 - 1;
+    if (i1 < 1 || j1 < 1 || j2 > sb->ne) return NAN; /* reference: out of bounds */
+    double x1 = (double)(i1 - 1) / d, x2 = (double)(i2 - 1) / d;
+    double y1 = sb->log_energy[j1 - 1], y2 = sb->log_energy[j2 - 1];
+    const double* u = sb->data;
+#define U_(i, j) u[(size_t)((i) - 1) + (size_t)ncum * (size_t)((j) - 1)]
+    double A = (x2 - x1) * (y2 - y1);
+    double S = U_(i1, j1) * (x2 - x) * (y2 - y) + U_(i1, j2) * (x - x1) * (y2 - y);
+    S = S + U_(i2, j1) * (x2 - x) * (y - y1);
+    S = S + U_(i2, j2) * (x - x1) * (y - y1);
+#undef U_
+    return exp(S / A);
+}
+
+/* src/compton.jl:119-144 */
+void or_compton_sample(double eng, or_rng* g, double* E1, double* cost) {
+    double e0 = MC2 / (MC2 + 2.0 * eng);
+    double a1 = -log(e0);
+    double a2 = (1.0 - e0 * e0) / 2.0;
+    double t, eps;
+    for (;;) {
+        if (or_rand(g) < a1 / (a1 + a2)) eps = exp(-or_rand(g) * a1);
+        else eps = sqrt(e0 * e0 + (1.0 - e0 * e0) * or_rand(g));
+        t = MC2 * (1.0 - eps) / (eps * eng);
+        double gg = 1.0 - eps / (1.0 + eps * eps) * t * (2.0 - t);
+        if (or_rand(g) < gg) break;
+        if (g->exhausted) break;
+    }
+    *cost = 1.0 - t;
+    *E1 = eps * eng;
+}
+
+/* src/photo_electric.jl:78-99 */
+double or_sauter_sample(double ee, or_rng* g) {
+    double gm = 1.0 + ee / MC2;
+    double b = sqrt(1.0 - 1.0 / (gm * gm));
+    double A = 1.0 / b - 1.0;
+    double k2 = b * gm * (gm - 1.0) * (gm - 2.0) / 2.0;
+    double g0 = 2.0 * (1.0 / A + k2);
+    double nu;
+    for (;;) {
+        double xi = or_rand(g);
+        nu = 2.0 * A / ((A + 2.0) * (A + 2.0) - 4.0 * xi) * (2.0 * xi + (A + 2.0) * sqrt(xi));
+        double xi1 = or_rand(g);
+        double gn = (2.0 - nu) * (1.0 / (A + nu) + k2);
+        if (xi1 * g0 < gn) break;
+        if (g->exhausted) break;
+    }
+    return 1.0 - nu;
+}
+
+/* src/bethe_heitler.jl:253-293 */
+static double screen1(double d) {
+    return d > 1.4 ? 42.038 - 8.29 * log(d + 0.958) : 42.184 - d * (7.444 - 1.623 * d);
+}
+static double screen2(double d) {
+    return d > 1.4 ? 42.038 - 8.29 * log(d + 0.958) : 41.326 - d * (5.848 - 0.902 * d);
+}
+/* src/bethe_heitler.jl:253-261 (note: alpha, not alpha*Z, as in the reference) */
+static double bh_fc(void) {
+    double az = ALPHA;
+    double az2 = az * az, az4 = az2 * az2, az6 = az4 * az2;
+    double f1 = (((1.0 / (1.0 + az2) + 0.20206) - 0.0369 * az2) + 0.0083 * az4) - 0.0020 * az6;
+    return f1 * az2;
+}
+
+/* src/bethe_heitler.jl:185-246; returns (ekin, pkin) in the reference's naming */
+void or_bh_sample(int Z, double eng, or_rng* g, double* ekin, double* pkin) {
+    double e0 = MC2 / eng;
+    double eps;
+    if (eng < 2e6 * EV) {
+        eps = e0 + (0.5 - e0) * or_rand(g);
+    } else {
+        double d0 = 136.0 * e0 / pow((double)Z, 1.0 / 3.0);
+        double FZ = 8.0 * log((double)Z) / 3.0;
+        if (eng > 50e6 * EV) FZ += 8.0 * bh_fc();
+        double dmin = 4.0 * d0;
+        double dmax = exp((42.24 - FZ) / 8.368) - 0.952;
+        double ep = (1.0 - sqrt(1.0 - dmin / dmax)) / 2.0;
+        double emin = e0 > ep ? e0 : ep;
+        double erange = 0.5 - emin;
+        double F10 = screen1(dmin) - FZ, F20 = screen2(dmin) - FZ;
+        double NF1 = jmax(F10 * (erange * erange), 0.0);
+        double NF2 = jmax(1.5 * F20, 0.0);
+        double NC = NF1 / (NF1 + NF2);
+        for (;;) {
+            int acc;
+            if (NC > or_rand(g)) {
+                eps = 0.5 - erange * pow(or_rand(g), 1.0 / 3.0);
+                double d = d0 / (eps * (1.0 - eps));
+                acc = or_rand(g) < (screen1(d) - FZ) / F10;
+            } else {
+                eps = emin + erange * or_rand(g);
+                double d = d0 / (eps * (1.0 - eps));
+                acc = or_rand(g) < (screen2(d) - FZ) / F20;
+            }
+            if (acc || g->exhausted) break;
+        }
+    }
+    double etot, ptot;
+    if (or_rand(g) < 0.5) { /* rand(Bool), src/bethe_heitler.jl:233 */
+        etot = (1.0 - eps) * eng; ptot = eps * eng;
+    } else {
+        ptot = (1.0 - eps) * eng; etot = eps * eng;
+    }
+    *ekin = jmax(0.0, etot - MC2);
+    *pkin = jmax(0.0, ptot - MC2);
+}
+
+/* src/anihilation.jl:39-59 */
+double or_annih_sample(double eng, or_rng* g) {
+    double gm = 1.0 + eng / MC2;
+    double sq = sqrt((gm - 1.0) / (gm + 1.0));
+    double emax = (1.0 + sq) / 2.0, emin = (1.0 - sq) / 2.0;
+    double eps;
+    for (;;) {
+        eps = emin * pow(emax / emin, or_rand(g));
+        double gg = 1.0 - eps + (2.0 * gm * eps - 1.0) / (eps * ((gm + 1.0) * (gm + 1.0)));
+        if (or_rand(g) < gg) break;
+        if (g->exhausted) break;
+    }
+    return eps;
+}
+
+/* ---------------------------------------------------------------- collide */
+static void new_state(or_state* d, const double x[3], const double p[3], double w, double t,
+                      uint64_t uid, int swarm, or_rng* g) {
+    d->x[0] = x[0]; d->x[1] = x[1]; d->x[2] = x[2];
+    d->p[0] = p[0]; d->p[1] = p[1]; d->p[2] = p[2];
+    d->w = w; d->t = t;
+    d->s = nextcoll(g); /* constructor default, e.g. src/electron.jl:32 */
+    d->r = 0.0; d->active = 1; d->uid = uid; d->swarm = swarm;
+}
+
+/* Lehtinen angles + turn for the two-electron processes (src/rbeb.jl:65-79,
+ * src/moller.jl:218-232, src/bhaba.jl:311-325) */
+static void two_body(const or_state* st, double E0, double E1, double E2, double phi,
+                     double p1v[3], double p2v[3]) {
+    double p1 = sqrt(E1 * E1 + 2.0 * MC2 * E1) / C;
+    double p2 = sqrt(E2 * E2 + 2.0 * MC2 * E2) / C;
+    double c1 = sqrt(E1 * (E0 + 2.0 * MC2) / (E0 * (E1 + 2.0 * MC2)));
+    double c2 = sqrt(E2 * (E0 + 2.0 * MC2) / (E0 * (E2 + 2.0 * MC2)));
+    or_turn(st->p, c1, phi, p1, p1v);
+    or_turn(st->p, c2, -phi, p2, p2v);
+}
+
+int or_collide(const or_proc* pr, int species, const or_state* st, double eng, uint64_t seed,
+               uint32_t step, or_rng* g, or_outcome* out) {
+    memset(out, 0, sizeof(*out));
+    double Z = pr->zf != 0.0 ? pr->zf : (double)pr->Z;
+    int err = 0;
+    switch (pr->kind) {
+    case OR_ELASTIC: { /* src/relativistic_coulomb.jl:10-23 */
+        double phi = TWO_PI * or_rand(g);
+        double v[3];
+        velocity(species, st->p, v);
+        double beta = norm3(v) / C;
+        double a = 1.3413 * pow(Z, -1.0 / 3.0) * A0;
+        double al = HBAR * HBAR / (4.0 * dot3(st->p, st->p) * (a * a));
+        double cost = or_sample_rel_sr(al, beta, g);
+        double pn[3];
+        or_turn(st->p, cost, phi, norm3(st->p), pn);
+        out->kind = OR_OUT_STATECHANGE;
+        new_state(&out->st[0], st->x, pn, st->w, st->t, st->uid, st->swarm, g);
+        out->species[0] = species;
+        break;
+    }
+    case OR_RBEB: { /* src/rbeb.jl:54-80 */
+        double E0 = eng;
+        double E2 = or_rbeb_sample(E0, pr->B, g);
+        double E1 = E0 - E2 - pr->B;
+        if (!(E2 < E1)) err = 1;
+        double phi = TWO_PI * or_rand(g);
+        double p1v[3], p2v[3];
+        two_body(st, E0, E1, E2, phi, p1v, p2v);
+        out->kind = OR_OUT_NEW;
+        new_state(&out->st[0], st->x, p1v, st->w, st->t, st->uid, st->swarm, g);
+        new_state(&out->st[1], st->x, p2v, st->w, st->t, 0, st->swarm, g);
+        out->species[0] = OR_ELECTRON; out->species[1] = OR_ELECTRON;
+        out->nnew = 1;
+        break;
+    }
+    case OR_MOLLER: { /* src/moller.jl:13-37 */
+        double E2 = or_moller_sample(pr->tcut, eng, g);
+        double E0 = eng, E1 = E0 - E2;
+        double phi = TWO_PI * or_rand(g);
+        double p1v[3], p2v[3];
+        two_body(st, E0, E1, E2, phi, p1v, p2v);
+        out->kind = OR_OUT_NEW;
+        new_state(&out->st[0], st->x, p1v, st->w, st->t, st->uid, st->swarm, g);
+        new_state(&out->st[1], st->x, p2v, st->w, st->t, 0, st->swarm, g);
+        out->species[0] = OR_ELECTRON; out->species[1] = OR_ELECTRON;
+        out->nnew = 1;
+        break;
+    }
+    case OR_BHABHA: { /* src/bhaba.jl:9-33 */
+        double E2 = or_bhabha_sample(pr->tcut, eng, g);
+        double E0 = eng, E1 = E0 - E2;
+        double phi = TWO_PI * or_rand(g);
+        double p1v[3], p2v[3];
+        two_body(st, E0, E1, E2, phi, p1v, p2v);
+        out->kind = OR_OUT_NEW;
+        new_state(&out->st[0], st->x, p1v, st->w, st->t, st->uid, st->swarm, g);
+        new_state(&out->st[1], st->x, p2v, st->w, st->t, 0, st->swarm, g);
+        out->species[0] = OR_POSITRON; out->species[1] = OR_ELECTRON;
+        out->nnew = 1;
+        break;
+    }
+    case OR_SB: { /* src/seltzer.jl:67-90 */
+        double k = eng * or_sb_sample(pr->sb, eng, g);
+        if (!(k < eng)) err = 2;
+        double pnorm = k / C;
+        double cost = or_sample_tsai(eng, g);
+        double phi = TWO_PI * or_rand(g);
+        double pph[3], pe[3];
+        or_turn(st->p, cost, phi, pnorm, pph);
+        pe[0] = st->p[0] - pph[0]; pe[1] = st->p[1] - pph[1]; pe[2] = st->p[2] - pph[2];
+        out->kind = OR_OUT_NEW;
+        new_state(&out->st[0], st->x, pe, st->w, st->t, st->uid, st->swarm, g);
+        new_state(&out->st[1], st->x, pph, st->w, st->t, 0, st->swarm, g);
+        out->species[0] = OR_ELECTRON; out->species[1] = OR_PHOTON;
+        out->nnew = 1;
+        break;
+    }
+    case OR_COMPTON:
+    case OR_KN_COMPTON: { /* src/compton.jl:9-28 */
+        double E1, cost;
+        or_compton_sample(eng, g, &E1, &cost);
+        double phi = TWO_PI * or_rand(g);
+        double pg[3], pe[3];
+        or_turn(st->p, cost, phi, E1 / C, pg);
+        pe[0] = st->p[0] - pg[0]; pe[1] = st->p[1] - pg[1]; pe[2] = st->p[2] - pg[2];
+        out->kind = OR_OUT_NEW;
+        new_state(&out->st[0], st->x, pg, st->w, st->t, st->uid, st->swarm, g);
+        new_state(&out->st[1], st->x, pe, st->w, st->t, 0, st->swarm, g);
+        out->species[0] = OR_PHOTON; out->species[1] = OR_ELECTRON;
+        out->nnew = 1;
+        break;
+    }
+    case OR_PHOTO: { /* src/photo_electric.jl:38-52, :60-76 */
+        const or_pe* pe = pr->pe;
+        double b = pe->binding[0];
+        for (int q = 0; q < pe->nshell; q++) {
+            b = pe->binding[q];
+            if (eng > b) break;
+        }
+        if (!(eng > b)) err = 3;
+        double ee = eng - b;
+        double cost = or_sauter_sample(ee, g);
+        double phi = TWO_PI * or_rand(g);
+        double pn = or_momentum_norm_from_kin(OR_ELECTRON, ee);
+        double p[3];
+        or_turn(st->p, cost, phi, pn, p);
+        out->kind = OR_OUT_REPLACE;
+        new_state(&out->st[1], st->x, p, st->w, st->t, 0, st->swarm, g);
+        out->species[1] = OR_ELECTRON;
+        out->nnew = 1;
+        break;
+    }
+    case OR_BH: { /* src/bethe_heitler.jl:5-25 (names swapped as in the reference) */
+        double pkin, ekin;
+        or_bh_sample(pr->Z, eng, g, &pkin, &ekin);
+        double phi = TWO_PI * or_rand(g);
+        double cost = or_sample_tsai(ekin, g);
+        double pn = or_momentum_norm_from_kin(OR_ELECTRON, ekin);
+        double pe[3], pp[3];
+        or_turn(st->p, cost, phi, pn, pe);
+        cost = or_sample_tsai(pkin, g);
+        pn = or_momentum_norm_from_kin(OR_POSITRON, pkin);
+        or_turn(st->p, cost, phi, pn, pp);
+        out->kind = OR_OUT_REPLACE_PAIR;
+        new_state(&out->st[1], st->x, pe, st->w, st->t, 0, st->swarm, g);
+        new_state(&out->st[2], st->x, pp, st->w, st->t, 0, st->swarm, g);
+        out->species[1] = OR_ELECTRON; out->species[2] = OR_POSITRON;
+        out->nnew = 2;
+        break;
+    }
+    case OR_ANNIH: { /* src/anihilation.jl:6-23, :61-67 */
+        double eps = or_annih_sample(eng, g);
+        double gm = 1.0 + eng / MC2;
+        double cost = (eps * (gm + 1.0) - 1.0) / (eps * sqrt(gm * gm - 1.0));
+        double phi = TWO_PI * or_rand(g);
+        double etot = eng + 2.0 * MC2;
+        double pan = eps * etot / C;
+        double pa[3], pb[3];
+        or_turn(st->p, cost, phi, pan, pa);
+        pb[0] = st->p[0] - pa[0]; pb[1] = st->p[1] - pa[1]; pb[2] = st->p[2] - pa[2];
+        out->kind = OR_OUT_REPLACE_PAIR;
+        new_state(&out->st[1], st->x, pa, st->w, st->t, 0, st->swarm, g);
+        new_state(&out->st[2], st->x, pb, st->w, st->t, 0, st->swarm, g);
+        out->species[1] = OR_PHOTON; out->species[2] = OR_PHOTON;
+        out->nnew = 2;
+        break;
+    }
+    default:
+        out->kind = OR_OUT_NULL;
+        return -1;
+    }
+    /* uids of the created particles: keyed by the event's final draw index */
+    for (int q = 1; q <= 2; q++)
+        if ((q == 1 && out->nnew >= 1) || (q == 2 && out->nnew >= 2))
+            out->st[q].uid = or_child_uid(seed, st->uid, step, g->draw, q - 1);
+    return err;
+}
+
+/* ---------------------------------------------------------------- fields and push */
+/* src/field.jl:4-52 */
+void or_field_eval(const or_field_fn* f, const double x[3], double t, double out[3]) {
+    (void)t;
+    switch (f->kind) {
+    case OR_FIELD_HOMOGENEOUS:
+        out[0] = f->v[0]; out[1] = f->v[1]; out[2] = f->v[2];
+        return;
+    case OR_FIELD_DOUBLELAYER:
+        if (f->z1 < x[2] && x[2] < f->z2) { out[0] = f->v[0]; out[1] = f->v[1]; out[2] = f->v[2]; }
+        else { out[0] = 0.0; out[1] = 0.0; out[2] = 0.0; }
+        return;
+    case OR_FIELD_STEP:
+        if (x[2] < f->z1) { out[0] = f->v[0]; out[1] = f->v[1]; out[2] = f->v[2]; }
+        else { out[0] = f->v2[0]; out[1] = f->v2[1]; out[2] = f->v2[2]; }
+        return;
+    case OR_FIELD_CONFINED: {
+        double xx = x[0], yy = x[1], zz = x[2];
+        double g = exp(-(xx * xx / (2.0 * (f->sx * f->sx)) + yy * yy / (2.0 * (f->sy * f->sy)) +
+                         zz * zz / (2.0 * (f->sz * f->sz))));
+        out[0] = -(f->ez0 * g * xx * zz) / (f->sx * f->sx);
+        out[1] = -(f->ez0 * g * yy * zz) / (f->sy * f->sy);
+        out[2] = f->ez0 * g - (f->ez0 * g * (zz * zz)) / (f->sz * f->sz);
+        return;
+    }
+    default:
+        out[0] = 0.0; out[1] = 0.0; out[2] = 0.0;
+    }
+}
+
+/* src/field.jl:62-70 */
+static void force(int species, const or_field* f, const double x[3], const double p[3],
+                  double t, double out[3]) {
+    if (species == OR_PHOTON) { out[0] = 0.0; out[1] = 0.0; out[2] = 0.0; return; }
+    double e[3], b[3], v[3];
+    or_field_eval(&f->e, x, t, e);
+    or_field_eval(&f->b, x, t, b);
+    velocity(species, p, v);
+    double q = (species == OR_ELECTRON ? -1.0 : 1.0) * QE;
+    double cx = v[1] * b[2] - v[2] * b[1];
+    double cy = v[2] * b[0] - v[0] * b[2];
+    double cz = v[0] * b[1] - v[1] * b[0];
+    out[0] = q * (e[0] + cx); out[1] = q * (e[1] + cy); out[2] = q * (e[2] + cz);
+}
+
+/* RK2Pusher advance_particle, src/pusher.jl:41-63 */
+void or_push(int species, const or_field* f, or_state* y, double dt) {
+    double v1[3], f1[3], v2[3], f2[3], x2[3], p2[3];
+    velocity(species, y->p, v1);
+    force(species, f, y->x, y->p, y->t, f1);
+    double dt2 = 2.0 * dt;
+    for (int d = 0; d < 3; d++) {
+        x2[d] = y->x[d] + dt2 * v1[d] / 3.0;
+        p2[d] = y->p[d] + dt2 * f1[d] / 3.0;
+    }
+    double t2 = y->t + dt2 / 3.0;
+    velocity(species, p2, v2);
+    force(species, f, x2, p2, t2, f2);
+    for (int d = 0; d < 3; d++) {
+        y->x[d] = y->x[d] + dt * (v1[d] / 4.0 + 3.0 * v2[d] / 4.0);
+        y->p[d] = y->p[d] + dt * (f1[d] / 4.0 + 3.0 * f2[d] / 4.0);
+    }
+    y->t = y->t + dt;
+}
+
+/* ---------------------------------------------------------------- populations */
+int or_world_init(or_world* w, uint64_t seed, int nthreads) {
+    memset(w, 0, sizeof(*w));
+    w->seed = seed;
+    w->nthreads = nthreads;
+    w->nswarm = 1;
+    return 0;
+}
+
+int or_pop_setup(or_world* w, int species, or_state* buf, size_t cap, const or_table* table,
+                 double energy_cut, int tracked) {
+    or_pop* p = &w->pop[species];
+    p->parts = buf; p->cap = cap; p->n = 0; p->iup = 0; p->table = table;
+    p->energy_cut = energy_cut; p->species = species; p->tracked = tracked;
+    return 0;
+}
+
+void or_set_field(or_world* w, int swarm, const or_field* f) {
+    w->field[swarm] = *f;
+    if (swarm + 1 > w->nswarm) w->nswarm = swarm + 1;
+}
+
+/* setr!, src/collisions.jl:63-74 */
+void or_setr(or_world* w, int species, or_state* s) {
+    const or_pop* P = &w->pop[species];
+    double eng = or_kinenergy(species, s->p);
+    if (eng < P->energy_cut) { s->r = 0.0; return; }
+    double t[3];
+    int i = clamp_interval(P->table, or_precheb(eng, P->table->k, P->table->xmax, t));
+    int sw = P->table->nswarm > 1 ? s->swarm : 0;
+    s->r = or_ratebound(P->table, sw, i, t);
+}
+
+/* add_particle!, src/population.jl:103-113 (+ untracked species are tallied) */
+int or_add_particle(or_world* w, int species, const or_state* s) {
+    or_pop* P = &w->pop[species];
+    double eng = or_kinenergy(species, s->p);
+    if (eng <= P->energy_cut) return -1;
+    if (!P->tracked) {
+#pragma omp atomic
+        P->tally_n += 1.0;
+#pragma omp atomic
+        P->tally_w += s->w;
+        if (P->tally_hist && P->tally_nbins > 0) {
+            double lx = log(eng / P->tally_lo) / log(P->tally_hi / P->tally_lo);
+            int b = (int)floor(lx * P->tally_nbins);
+            if (b >= 0 && b < P->tally_nbins) {
+#pragma omp atomic
+                P->tally_hist[b] += s->w;
+            }
+        }
+        return -1;
+    }
+    size_t slot;
+#pragma omp atomic capture
+    slot = P->n++;
+    if (slot >= P->cap) {
+#pragma omp atomic
+        P->n--;
+        P->overflow = 1;
+        return -1;
+    }
+    P->parts[slot] = *s;
+    or_setr(w, species, &P->parts[slot]);
+    return (int)slot;
+}
+
+/* apply!, src/collisions.jl:83-133 */
+static void apply_outcome(or_world* w, int species, or_state* st, const or_outcome* o) {
+    switch (o->kind) {
+    case OR_OUT_STATECHANGE:
+    case OR_OUT_NEW:
+        *st = o->st[0];
+        or_setr(w, species, st);
+        if (o->kind == OR_OUT_NEW) or_add_particle(w, o->species[1], &o->st[1]);
+        break;
+    case OR_OUT_REPLACE:
+        st->active = 0;
+        or_add_particle(w, o->species[1], &o->st[1]);
+        break;
+    case OR_OUT_REPLACE_PAIR:
+        st->active = 0;
+        or_add_particle(w, o->species[1], &o->st[1]);
+        or_add_particle(w, o->species[2], &o->st[2]);
+        break;
+    default: break;
+    }
+}
+
+/* do_one_collision!, src/collisions.jl:142-199 */
+static void do_one_collision(or_world* w, int species, or_state* st, or_rng* g) {
+    const or_pop* P = &w->pop[species];
+    if (st->r == 0.0) return;
+    double eng = or_kinenergy(species, st->p);
+    if (!(eng >= P->energy_cut)) return;
+    const or_table* tb = P->table;
+    double t[3];
+    int i = clamp_interval(tb, or_precheb(eng, tb->k, tb->xmax, t));
+    double xi = or_rand(g) * st->r;
+    for (int j = 0; j < tb->nprocs; j++) {
+        double nu = or_rate(tb, j, i, t);
+        if (nu > xi) {
+            or_outcome o;
+            or_collide(&tb->procs[j], species, st, eng, w->seed, w->step, g, &o);
+#pragma omp atomic
+            w->counts[species][j + 1]++;
+            apply_outcome(w, species, st, &o);
+            return;
+        }
+        xi -= nu;
+    }
+    if (!(xi >= 0.0)) {
+#pragma omp atomic
+        w->ratebound_violations++;
+    }
+#pragma omp atomic
+    w->counts[species][0]++;
+    or_setr(w, species, st);
+    st->s = nextcoll(g);
+}
+
+/* body of the @batch loop, src/mixed_population.jl:61-87 */
+static void advance_particle_loop(or_world* w, int species, size_t i, double tfinal) {
+    or_pop* P = &w->pop[species];
+    or_state* l = &P->parts[i];
+    if (!l->active) return;
+    or_rng g;
+    or_rng_philox(&g, w->seed, l->uid, w->step);
+    const or_field* f = &w->field[w->nswarm > 1 ? l->swarm : 0];
+    double trem = tfinal - l->t;
+    long iters = 0;
+    while (trem > EPS64 && l->active) {
+        if (++iters > (1L << 30)) break; /* guard for the tests; never reached in practice */
+        double tnext = l->s / l->r;
+        double dt;
+        int collides;
+        if (trem > tnext) { dt = tnext; collides = 1; }
+        else { dt = trem; collides = 0; l->s -= dt * l->r; }
+        or_push(species, f, l, dt);
+        if (collides && l->active) do_one_collision(w, species, l, &g);
+        trem -= dt;
+    }
+}
+
+/* advance!, src/mixed_population.jl:38-47, :56-93, :99-112 */
+int or_advance(or_world* w, double tfinal) {
+    for (int s = 0; s < OR_NSPECIES; s++) {
+        or_pop* P = &w->pop[s];
+        if (!P->parts) continue;
+        P->iup = 0;
+        long n = (long)P->n;
+#pragma omp parallel for schedule(static) num_threads(w->nthreads > 0 ? w->nthreads : 1)
+        for (long i = 0; i < n; i++)
+            if (P->parts[i].active) or_setr(w, s, &P->parts[i]);
+    }
+    for (;;) {
+        size_t total = 0;
+        for (int s = 0; s < OR_NSPECIES; s++) {
+            or_pop* P = &w->pop[s];
+            if (!P->parts) continue;
+            long lo = (long)P->iup, hi = (long)P->n;
+#pragma omp parallel for schedule(static) num_threads(w->nthreads > 0 ? w->nthreads : 1)
+            for (long i = lo; i < hi; i++) advance_particle_loop(w, s, (size_t)i, tfinal);
+            total += (size_t)(hi - lo);
+            P->iup = (size_t)hi;
+        }
+        w->passes++;
+        if (total == 0) break;
+    }
+    return 0;
+}
+
+/* repack!, src/population.jl:229-259 (swap-from-end) */
+void or_repack(or_pop* P) {
+    long l = (long)P->n;
+    if (l == 0) return;
+    while (l > 0 && !P->parts[l - 1].active) l--;
+    if (l == 0) { P->n = 0; return; }
+    long i = 1;
+    while (i <= l) {
+        if (!P->parts[i - 1].active) {
+            P->parts[i - 1] = P->parts[l - 1];
+            l--;
+            while (!P->parts[l - 1].active) l--;
+        }
+        i++;
+    }
+    P->n = (size_t)l;
+}
+
+/* droplow!, src/population.jl:273-284 */
+void or_droplow(or_world* w, int species) {
+    or_pop* P = &w->pop[species];
+    if (!P->parts) return;
+    for (size_t i = 0; i < P->n; i++)
+        if (P->parts[i].active && or_kinenergy(species, P->parts[i].p) < P->energy_cut)
+            P->parts[i].active = 0;
+    or_repack(P);
+}
+
+/* nactives, src/population.jl:89-97 */
+size_t or_nactives(const or_pop* P) {
+    size_t c = 0;
+    for (size_t i = 0; i < P->n; i++) c += P->parts[i].active ? 1 : 0;
+    return c;
+}
+
+/* weight, meanenergy, spread: src/population.jl:130-198 */
+void or_stats(const or_pop* P, double* out) {
+    double na = 0, wsum = 0, esum = 0, xs[3] = {0, 0, 0}, x2 = 0;
+    for (size_t i = 0; i < P->n; i++) {
+        const or_state* s = &P->parts[i];
+        if (!s->active) continue;
+        na += 1.0;
+        wsum += s->w;
+        esum += s->w * or_kinenergy(P->species, s->p);
+        for (int d = 0; d < 3; d++) xs[d] += s->w * s->x[d];
+        x2 += s->w * dot3(s->x, s->x);
+    }
+    out[0] = na; out[1] = wsum; out[2] = esum / wsum;
+    for (int d = 0; d < 3; d++) out[3 + d] = xs[d] / wsum;
+    double xm2 = (out[3] * out[3] + out[4] * out[4]) + out[5] * out[5];
+    out[6] = sqrt(fabs(x2 / wsum - xm2));
+}
+
+/* Initial swarm (build-defined draw order): energy (if log-uniform), direction
+ * (randsphere, src/util.jl:4-12), then the constructor's s = nextcoll(). Stream step
+ * 0xFFFFFFFF is reserved for initialisation. */
+void or_init_swarm(or_state* out, size_t n, int species, uint64_t seed, uint64_t uid0,
+                   int swarm, double e1, double e2, int loguniform) {
+    for (size_t i = 0; i < n; i++) {
+        or_rng g;
+        or_rng_philox(&g, seed, uid0 + i, 0xFFFFFFFFu);
+        double E = e1;
+        if (loguniform) {
+            double la = log(e1), lb = log(e2);
+            E = exp(la + or_rand(&g) * (lb - la));
+        }
+        double d[3];
+        or_randsphere(&g, d);
+        double pn = or_momentum_norm_from_kin(species, E);
+        or_state* s = &out[i];
+        s->x[0] = 0.0; s->x[1] = 0.0; s->x[2] = 0.0;
+        s->p[0] = pn * d[0]; s->p[1] = pn * d[1]; s->p[2] = pn * d[2];
+        s->w = 1.0; s->t = 0.0;
+        s->s = nextcoll(&g);
+        s->r = 0.0; s->active = 1; s->uid = uid0 + i; s->swarm = swarm;
+    }
+}

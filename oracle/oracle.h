@@ -1,0 +1,220 @@
+/*
+ * oracle.h -- CPU restatement of Particulator.jl's null-collision Monte Carlo step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, the Python host
+ * package) may include, link or call this code.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg use it, as the checker and as the CPU baseline.
+ *
+ * Every function restates the reference's Julia code, keeping its evaluation order
+ * (left-fold sums, no FMA contraction: build with -ffp-contract=off).  Each definition in
+ * oracle.c cites the reference file:line it follows.  The random-number source is the
+ * build's own uid-keyed Philox4x32-10 stream (the reference draws from Julia's task-local
+ * Xoshiro, which is not reproducible under threads), or an injected array of uniforms.
+ *
+ * Pinning status of each part is listed at the top of oracle.c.
+ */
+#ifndef PT_ORACLE_H
+#define PT_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* species and process kinds (same numbering as the product header) */
+enum { OR_ELECTRON = 0, OR_PHOTON = 1, OR_POSITRON = 2, OR_NSPECIES = 3 };
+enum {
+    OR_NULL = 0, OR_ELASTIC = 1, OR_SB = 2, OR_RBEB = 3, OR_MOLLER = 4, OR_COMPTON = 5,
+    OR_PHOTO = 6, OR_BH = 7, OR_BHABHA = 8, OR_ANNIH = 9, OR_KN_COMPTON = 10
+};
+/* outcome kinds (src/collisions.jl:11-55) */
+enum { OR_OUT_NULL = 0, OR_OUT_STATECHANGE = 1, OR_OUT_NEW = 2, OR_OUT_REPLACE = 3,
+       OR_OUT_REPLACE_PAIR = 4, OR_OUT_SKIPPED = 5 };
+/* field kinds (src/field.jl) */
+enum { OR_FIELD_NONE = 0, OR_FIELD_HOMOGENEOUS = 1, OR_FIELD_DOUBLELAYER = 2,
+       OR_FIELD_STEP = 3, OR_FIELD_CONFINED = 4 };
+
+typedef struct {
+    double c, c2, m_e, e, alpha, hbar, eps0, N_A, eV, kilo, centi, k_B, atm;
+    double r_e, a_0, mc2, mc, nair;
+} or_consts;
+
+typedef struct {
+    double x[3], p[3], w, t, s, r;
+    uint64_t uid;
+    int32_t active;
+    int32_t swarm;
+} or_state;
+
+/* Uniform source: Philox4x32-10 keyed by (seed), counter (uid, step, draw/2), or an
+ * injected array consumed in order. */
+typedef struct {
+    uint32_t key[2];
+    uint64_t uid;
+    uint32_t step;
+    uint32_t draw;          /* next draw index */
+    uint32_t cblock;        /* cached block index (+1), 0 = none */
+    double cache[2];
+    const double* inj;      /* injected uniforms (mode 1) */
+    size_t ninj, pos;
+    int mode;               /* 0 = philox, 1 = injected */
+    int exhausted;          /* injected stream ran out */
+} or_rng;
+
+/* Seltzer-Berger sampling table (src/seltzer.jl:9-64). data is column-major [ncum, ne]. */
+typedef struct {
+    int Z, ne, ncum;
+    const double* log_energy;
+    const double* totalcs;
+    const double* data;
+} or_sb;
+
+/* Photoelectric data (src/photo_electric.jl:7-35): coeffs[i + 4*j]. */
+typedef struct {
+    int Z, nint, nshell;
+    const double* left_energy;
+    const double* coeffs;
+    const double* binding;
+} or_pe;
+
+typedef struct {
+    int kind, Z, N, pad;
+    double B, U, tcut;   /* RBEB: B, U; Moller/Bhabha: tcut */
+    double coef;         /* density of scatterers (1/m^3) */
+    double zf;           /* non-integer Z (annihilation allows effective Z) */
+    const or_sb* sb;
+    const or_pe* pe;
+} or_proc;
+
+/* Chebyshev collision table (src/collision_table.jl:63-106). rate[o + order*(j + L*i)],
+ * ratebound[o + order*i] (+ order*(k+1)*swarm for per-swarm bounds). */
+typedef struct {
+    int k, order, nprocs, nswarm;
+    double xmax;
+    const double* rate;
+    const double* ratebound;
+    const or_proc* procs;
+} or_table;
+
+typedef struct {
+    int kind;
+    double v[3], v2[3];     /* homogeneous / double layer value, step field v1,v2 */
+    double z1, z2;          /* double layer bounds or step z (z1) */
+    double sx, sy, sz, ez0; /* confined double layer */
+} or_field_fn;
+
+typedef struct {
+    or_field_fn e, b;
+} or_field;
+
+typedef struct {
+    int kind;               /* OR_OUT_* */
+    int nnew;
+    int species[3];
+    or_state st[3];         /* st[0] = new primary state (if kept) */
+} or_outcome;
+
+/* one population (src/population.jl:7-21) */
+typedef struct {
+    or_state* parts;
+    size_t cap, n, iup;
+    const or_table* table;
+    double energy_cut;
+    int species, tracked, overflow, pad;
+    /* tallies of untracked births: count, weight, energy histogram */
+    double tally_w, tally_n;
+    double* tally_hist; int tally_nbins; double tally_lo, tally_hi;
+} or_pop;
+
+typedef struct {
+    or_pop pop[OR_NSPECIES];
+    or_field field[64];     /* per swarm */
+    int nswarm;
+    uint64_t seed;
+    uint32_t step;
+    int nthreads;
+    int64_t counts[OR_NSPECIES][40]; /* per species: null, then process j+1 */
+    int64_t ratebound_violations;
+    int64_t passes;
+} or_world;
+
+void or_get_consts(or_consts* out);
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double or_u01(uint64_t bits);
+void or_rng_philox(or_rng* g, uint64_t seed, uint64_t uid, uint32_t step);
+void or_rng_inject(or_rng* g, const double* u, size_t n);
+double or_rand(or_rng* g);
+uint64_t or_child_uid(uint64_t seed, uint64_t uid, uint32_t step, uint32_t draw, int k);
+
+/* kinematics */
+double or_kinenergy(int species, const double p[3]);
+double or_speed(int species, double eng);
+double or_momentum_norm_from_kin(int species, double kin);
+
+/* binary-interval Chebyshev (src/cheby.jl) */
+int or_precheb(double x, int k, double xmax, double t[3]);
+void or_interval(int k, double xmax, int i, double* l, double* r);
+double or_rate(const or_table* tb, int j, int i, const double t[3]);
+double or_ratebound(const or_table* tb, int swarm, int i, const double t[3]);
+int or_select(const or_table* tb, double eng, double u, double r, double* partials);
+
+/* cross sections (init-time, used to build tables) */
+double or_totalcs(const or_proc* p, double eng);
+double or_rbeb_dsdw(double W, double T, double B, double U);
+int or_chebfit_proc(const or_proc* p, int species, int k, double xmax, int order, double* out);
+int or_build_table(const or_proc* procs, int nprocs, int species, double Fdt, double safety,
+                   int k, double xmax, int order, double* rate_out, double* rb_out,
+                   int* perm_out, double* sortkey_out);
+int or_compratebound(const double* a, int nprocs, int species, double Fdt, double safety,
+                     int k, double xmax, int order, double* rb_out);
+int or_sb_build(int Z, const double* kfrac, int nx, const double* y, int ny,
+                const double* values, int ncum, double* log_energy, double* totalcs,
+                double* data);
+int or_sb_findcum(const double* x, const double* p, int n, const double* pcum, int npcum,
+                  double xmin, double xmax, double* out);
+int or_pe_build(int Z, double* left_energy, double* coeffs, double* binding, int* nint,
+                int* nshell);
+
+/* samplers (L3) */
+double or_sample_tsai(double T, or_rng* g);
+double or_sample_rel_sr(double alpha, double beta, or_rng* g);
+double or_rbeb_sample(double T, double B, or_rng* g);
+double or_moller_sample(double tcut, double eng, or_rng* g);
+double or_bhabha_sample(double tcut, double eng, or_rng* g);
+double or_sb_sample(const or_sb* sb, double T, or_rng* g);
+void or_compton_sample(double eng, or_rng* g, double* E1, double* cost);
+double or_sauter_sample(double ee, or_rng* g);
+void or_bh_sample(int Z, double eng, or_rng* g, double* ekin, double* pkin);
+double or_annih_sample(double eng, or_rng* g);
+void or_turn(const double u[3], double cost, double phi, double n, double out[3]);
+void or_randsphere(or_rng* g, double out[3]);
+
+/* one collision of process `proc` on a particle in `st` with energy eng (the collide methods of src/) */
+int or_collide(const or_proc* proc, int species, const or_state* st, double eng,
+               uint64_t seed, uint32_t step, or_rng* g, or_outcome* out);
+
+/* pusher (src/pusher.jl:37-63) */
+void or_push(int species, const or_field* f, or_state* s, double dt);
+void or_field_eval(const or_field_fn* f, const double x[3], double t, double out[3]);
+
+/* world / step */
+int or_world_init(or_world* w, uint64_t seed, int nthreads);
+int or_pop_setup(or_world* w, int species, or_state* buf, size_t cap, const or_table* table,
+                 double energy_cut, int tracked);
+void or_set_field(or_world* w, int swarm, const or_field* f);
+int or_add_particle(or_world* w, int species, const or_state* s);
+void or_setr(or_world* w, int species, or_state* s);
+int or_advance(or_world* w, double tfinal);
+void or_droplow(or_world* w, int species);
+void or_repack(or_pop* p);
+size_t or_nactives(const or_pop* p);
+void or_stats(const or_pop* p, double* out /* n_active, weight, meanE, cx,cy,cz, spread */);
+void or_init_swarm(or_state* out, size_t n, int species, uint64_t seed, uint64_t uid0,
+                   int swarm, double e1, double e2, int loguniform);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,515 @@
+"""ctypes wrapper of the CPU oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the product package.  It builds the oracle's own tables
+(cross sections, Chebyshev fits, rate bounds, Seltzer-Berger inverse CDFs) and wraps the
+reference-semantics step (src/mixed_population.jl:38-112, src/run.jl:1-29).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+ELECTRON, PHOTON, POSITRON = 0, 1, 2
+ELASTIC, SB, RBEB, MOLLER, COMPTON, PHOTO, BH, BHABHA, ANNIH, KN_COMPTON = range(1, 11)
+FIELD_NONE, FIELD_HOMOGENEOUS, FIELD_DOUBLELAYER, FIELD_STEP, FIELD_CONFINED = range(5)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+class Consts(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("c", "c2", "m_e", "e", "alpha", "hbar", "eps0", "N_A", "eV", "kilo", "centi",
+                 "k_B", "atm", "r_e", "a_0", "mc2", "mc", "nair")]
+
+
+class State(C.Structure):
+    _fields_ = [("x", C.c_double * 3), ("p", C.c_double * 3), ("w", C.c_double),
+                ("t", C.c_double), ("s", C.c_double), ("r", C.c_double),
+                ("uid", C.c_uint64), ("active", C.c_int32), ("swarm", C.c_int32)]
+
+
+STATE_DTYPE = np.dtype([("x", "f8", 3), ("p", "f8", 3), ("w", "f8"), ("t", "f8"), ("s", "f8"),
+                        ("r", "f8"), ("uid", "u8"), ("active", "i4"), ("swarm", "i4")])
+assert STATE_DTYPE.itemsize == C.sizeof(State)
+
+
+class Rng(C.Structure):
+    _fields_ = [("key", C.c_uint32 * 2), ("uid", C.c_uint64), ("step", C.c_uint32),
+                ("draw", C.c_uint32), ("cblock", C.c_uint32), ("cache", C.c_double * 2),
+                ("inj", C.POINTER(C.c_double)), ("ninj", C.c_size_t), ("pos", C.c_size_t),
+                ("mode", C.c_int), ("exhausted", C.c_int)]
+
+
+class SBT(C.Structure):
+    _fields_ = [("Z", C.c_int), ("ne", C.c_int), ("ncum", C.c_int),
+                ("log_energy", C.POINTER(C.c_double)), ("totalcs", C.POINTER(C.c_double)),
+                ("data", C.POINTER(C.c_double))]
+
+
+class PET(C.Structure):
+    _fields_ = [("Z", C.c_int), ("nint", C.c_int), ("nshell", C.c_int),
+                ("left_energy", C.POINTER(C.c_double)), ("coeffs", C.POINTER(C.c_double)),
+                ("binding", C.POINTER(C.c_double))]
+
+
+class Proc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("Z", C.c_int), ("N", C.c_int), ("pad", C.c_int),
+                ("B", C.c_double), ("U", C.c_double), ("tcut", C.c_double),
+                ("coef", C.c_double), ("zf", C.c_double),
+                ("sb", C.POINTER(SBT)), ("pe", C.POINTER(PET))]
+
+
+class Table(C.Structure):
+    _fields_ = [("k", C.c_int), ("order", C.c_int), ("nprocs", C.c_int), ("nswarm", C.c_int),
+                ("xmax", C.c_double), ("rate", C.POINTER(C.c_double)),
+                ("ratebound", C.POINTER(C.c_double)), ("procs", C.POINTER(Proc))]
+
+
+class FieldFn(C.Structure):
+    _fields_ = [("kind", C.c_int), ("v", C.c_double * 3), ("v2", C.c_double * 3),
+                ("z1", C.c_double), ("z2", C.c_double), ("sx", C.c_double),
+                ("sy", C.c_double), ("sz", C.c_double), ("ez0", C.c_double)]
+
+
+class Field(C.Structure):
+    _fields_ = [("e", FieldFn), ("b", FieldFn)]
+
+
+class Outcome(C.Structure):
+    _fields_ = [("kind", C.c_int), ("nnew", C.c_int), ("species", C.c_int * 3),
+                ("st", State * 3)]
+
+
+class Pop(C.Structure):
+    _fields_ = [("parts", C.POINTER(State)), ("cap", C.c_size_t), ("n", C.c_size_t),
+                ("iup", C.c_size_t), ("table", C.POINTER(Table)), ("energy_cut", C.c_double),
+                ("species", C.c_int), ("tracked", C.c_int), ("overflow", C.c_int),
+                ("pad", C.c_int), ("tally_w", C.c_double), ("tally_n", C.c_double),
+                ("tally_hist", C.POINTER(C.c_double)), ("tally_nbins", C.c_int),
+                ("tally_lo", C.c_double), ("tally_hi", C.c_double)]
+
+
+class World(C.Structure):
+    _fields_ = [("pop", Pop * 3), ("field", Field * 64), ("nswarm", C.c_int),
+                ("seed", C.c_uint64), ("step", C.c_uint32), ("nthreads", C.c_int),
+                ("counts", (C.c_int64 * 40) * 3), ("ratebound_violations", C.c_int64),
+                ("passes", C.c_int64)]
+
+
+_lib = None
+DP = C.POINTER(C.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        d, i, u32, u64, p = C.c_double, C.c_int, C.c_uint32, C.c_uint64, C.c_void_p
+        sig = {
+            "or_get_consts": (None, [C.POINTER(Consts)]),
+            "or_philox4x32_10": (None, [C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
+            "or_u01": (d, [u64]),
+            "or_rng_philox": (None, [C.POINTER(Rng), u64, u64, u32]),
+            "or_rng_inject": (None, [C.POINTER(Rng), DP, C.c_size_t]),
+            "or_rand": (d, [C.POINTER(Rng)]),
+            "or_child_uid": (u64, [u64, u64, u32, u32, i]),
+            "or_kinenergy": (d, [i, DP]),
+            "or_speed": (d, [i, d]),
+            "or_momentum_norm_from_kin": (d, [i, d]),
+            "or_precheb": (i, [d, i, d, DP]),
+            "or_interval": (None, [i, d, i, DP, DP]),
+            "or_select": (i, [C.POINTER(Table), d, d, d, DP]),
+            "or_rate": (d, [C.POINTER(Table), i, i, DP]),
+            "or_ratebound": (d, [C.POINTER(Table), i, i, DP]),
+            "or_totalcs": (d, [C.POINTER(Proc), d]),
+            "or_rbeb_dsdw": (d, [d, d, d, d]),
+            "or_chebfit_proc": (i, [C.POINTER(Proc), i, i, d, i, DP]),
+            "or_build_table": (i, [C.POINTER(Proc), i, i, d, d, i, d, i, DP, DP,
+                                   C.POINTER(i), DP]),
+            "or_sb_build": (i, [i, DP, i, DP, i, DP, i, DP, DP, DP]),
+            "or_sb_findcum": (i, [DP, DP, i, DP, i, d, d, DP]),
+            "or_pe_build": (i, [i, DP, DP, DP, C.POINTER(i), C.POINTER(i)]),
+            "or_sample_tsai": (d, [d, C.POINTER(Rng)]),
+            "or_sample_rel_sr": (d, [d, d, C.POINTER(Rng)]),
+            "or_rbeb_sample": (d, [d, d, C.POINTER(Rng)]),
+            "or_moller_sample": (d, [d, d, C.POINTER(Rng)]),
+            "or_bhabha_sample": (d, [d, d, C.POINTER(Rng)]),
+            "or_sb_sample": (d, [C.POINTER(SBT), d, C.POINTER(Rng)]),
+            "or_compton_sample": (None, [d, C.POINTER(Rng), DP, DP]),
+            "or_sauter_sample": (d, [d, C.POINTER(Rng)]),
+            "or_bh_sample": (None, [i, d, C.POINTER(Rng), DP, DP]),
+            "or_annih_sample": (d, [d, C.POINTER(Rng)]),
+            "or_turn": (None, [DP, d, d, d, DP]),
+            "or_randsphere": (None, [C.POINTER(Rng), DP]),
+            "or_collide": (i, [C.POINTER(Proc), i, C.POINTER(State), d, u64, u32,
+                               C.POINTER(Rng), C.POINTER(Outcome)]),
+            "or_push": (None, [i, C.POINTER(Field), C.POINTER(State), d]),
+            "or_field_eval": (None, [C.POINTER(FieldFn), DP, d, DP]),
+            "or_world_init": (i, [C.POINTER(World), u64, i]),
+            "or_pop_setup": (i, [C.POINTER(World), i, C.POINTER(State), C.c_size_t,
+                                 C.POINTER(Table), d, i]),
+            "or_set_field": (None, [C.POINTER(World), i, C.POINTER(Field)]),
+            "or_add_particle": (i, [C.POINTER(World), i, C.POINTER(State)]),
+            "or_setr": (None, [C.POINTER(World), i, C.POINTER(State)]),
+            "or_advance": (i, [C.POINTER(World), d]),
+            "or_droplow": (None, [C.POINTER(World), i]),
+            "or_repack": (None, [C.POINTER(Pop)]),
+            "or_nactives": (C.c_size_t, [C.POINTER(Pop)]),
+            "or_stats": (None, [C.POINTER(Pop), DP]),
+            "or_init_swarm": (None, [C.POINTER(State), C.c_size_t, i, u64, u64, i, d, d, i]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def dptr(a):
+    return a.ctypes.data_as(DP)
+
+
+def consts():
+    c = Consts()
+    lib().or_get_consts(C.byref(c))
+    return c
+
+
+CO = None
+
+
+def co():
+    global CO
+    if CO is None:
+        CO = consts()
+    return CO
+
+
+# ------------------------------------------------------------------ RNG
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().or_philox4x32_10(c, k, o)
+    return list(o)
+
+
+class Uniforms:
+    """A uniform stream: Philox (seed, uid, step) or an injected array."""
+
+    def __init__(self, seed=None, uid=0, step=0, inject=None):
+        self.rng = Rng()
+        if inject is not None:
+            self.buf = np.ascontiguousarray(inject, dtype=np.float64)
+            lib().or_rng_inject(C.byref(self.rng), dptr(self.buf), self.buf.size)
+        else:
+            lib().or_rng_philox(C.byref(self.rng), seed, uid, step)
+
+    def __call__(self):
+        return lib().or_rand(C.byref(self.rng))
+
+    @property
+    def draws(self):
+        return self.rng.draw
+
+    @property
+    def ref(self):
+        return C.byref(self.rng)
+
+
+# ------------------------------------------------------------------ data
+def read_g4_2d(path):
+    """RawG4Physics2DVector (src/util.jl:165-190): header `k nx ny`, x, y, ny x nx values."""
+    with open(path) as f:
+        k, nx, ny = (int(v) for v in f.readline().split())
+        x = np.array([float(v) for v in f.readline().split()])
+        y = np.array([float(v) for v in f.readline().split()])
+        vals = np.array([float(v) for v in f.read().split()]).reshape(ny, nx)
+    assert x.size == nx and y.size == ny
+    return x, y, vals
+
+
+DATA_DIR = os.path.join(os.path.dirname(HERE), "particulator.jl_amd", "data", "brem_SB")
+
+
+class OracleSB:
+    def __init__(self, Z, ncum=1000, path=None):
+        x, y, vals = read_g4_2d(path or os.path.join(DATA_DIR, "br%d" % Z))
+        self.Z = Z
+        self.x, self.y, self.vals = x, y, np.ascontiguousarray(vals)
+        ne = y.size
+        self.log_energy = np.zeros(ne)
+        self.totalcs = np.zeros(ne)
+        self.data = np.zeros((ne, ncum))  # row j = column j of the Julia matrix
+        lib().or_sb_build(Z, dptr(x), x.size, dptr(y), ne, dptr(self.vals), ncum,
+                          dptr(self.log_energy), dptr(self.totalcs), dptr(self.data))
+        self.set_struct()
+
+    def set_struct(self):
+        self.s = SBT(self.Z, self.log_energy.size, self.data.shape[1], dptr(self.log_energy),
+                     dptr(self.totalcs), dptr(self.data))
+
+    @classmethod
+    def from_arrays(cls, Z, log_energy, totalcs, data):
+        o = cls.__new__(cls)
+        o.Z = Z
+        o.log_energy = np.ascontiguousarray(log_energy, dtype=np.float64)
+        o.totalcs = np.ascontiguousarray(totalcs, dtype=np.float64)
+        o.data = np.ascontiguousarray(data, dtype=np.float64)
+        o.set_struct()
+        return o
+
+
+class OraclePE:
+    def __init__(self, Z):
+        self.Z = Z
+        self.left = np.zeros(16)
+        self.coeffs = np.zeros(64)
+        self.binding = np.zeros(16)
+        ni, ns = C.c_int(), C.c_int()
+        rc = lib().or_pe_build(Z, dptr(self.left), dptr(self.coeffs), dptr(self.binding),
+                               C.byref(ni), C.byref(ns))
+        assert rc == 0
+        self.nint, self.nshell = ni.value, ns.value
+        self.s = PET(Z, ni.value, ns.value, dptr(self.left), dptr(self.coeffs),
+                     dptr(self.binding))
+
+
+_SB_CACHE = {}
+_PE_CACHE = {}
+
+
+def sb(Z):
+    if Z not in _SB_CACHE:
+        _SB_CACHE[Z] = OracleSB(Z)
+    return _SB_CACHE[Z]
+
+
+def pe(Z):
+    if Z not in _PE_CACHE:
+        _PE_CACHE[Z] = OraclePE(Z)
+    return _PE_CACHE[Z]
+
+
+def make_proc(kind, Z=0, coef=1.0, B=0.0, U=0.0, N=0, tcut=0.0, zf=0.0, sbt=None, pet=None):
+    p = Proc()
+    p.kind, p.Z, p.N, p.B, p.U, p.tcut, p.coef, p.zf = kind, Z, N, B, U, tcut, coef, zf
+    if kind == SB:
+        t = sbt or sb(Z)
+        p.sb = C.pointer(t.s)
+        p._keep = t
+    if kind == PHOTO:
+        t = pet or pe(Z)
+        p.pe = C.pointer(t.s)
+        p._keep = t
+    return p
+
+
+def totalcs(p, eng):
+    return lib().or_totalcs(C.byref(p), eng)
+
+
+# ------------------------------------------------------------------ air (build-defined)
+# RBEB orbitals, src/rbeb.jl:25-48 (B eV, U eV, N)
+N2_ORBITALS = [(0.4095e3, 0.6033e3, 4), (41.72, 71.13, 2), (21.00, 63.18, 2),
+               (17.07, 44.30, 4), (15.58, 54.91, 2)]
+O2_ORBITALS = [(0.5438e3, 0.7962e3, 4), (46.19, 79.73, 2), (29.82, 90.92, 2),
+               (19.64, 59.89, 4), (19.79, 71.84, 2), (12.07, 84.88, 2)]
+
+
+def air_composition(fN2=0.781, fO2=0.210, fAr=0.009, T=273.0, P=None):
+    c = co()
+    n = (P if P is not None else c.atm) / (c.k_B * T)
+    return {"N2": fN2 * n, "O2": fO2 * n, "Ar": fAr * n}
+
+
+def air_processes(species, comp, tcut):
+    """Process lists of scripts/swarm.jl:104-139 extended with argon (Z=18, monatomic)."""
+    eV = co().eV
+    out = []
+    atoms = [(7, 2 * comp["N2"]), (8, 2 * comp["O2"])]
+    if comp.get("Ar", 0.0) > 0:
+        atoms.append((18, comp["Ar"]))
+    if species == ELECTRON:
+        out += [make_proc(ELASTIC, Z, coef) for Z, coef in atoms]
+        out += [make_proc(SB, Z, coef) for Z, coef in atoms]
+        out += [make_proc(RBEB, 0, comp["N2"], B * eV, U * eV, N) for B, U, N in N2_ORBITALS]
+        out += [make_proc(RBEB, 0, comp["O2"], B * eV, U * eV, N) for B, U, N in O2_ORBITALS]
+        if comp.get("Ar", 0.0) > 0:
+            out.append(make_proc(MOLLER, 18, comp["Ar"], tcut=tcut))
+    elif species == POSITRON:
+        out += [make_proc(ELASTIC, Z, coef) for Z, coef in atoms]
+        out += [make_proc(BHABHA, Z, coef, tcut=tcut) for Z, coef in atoms]
+        out += [make_proc(ANNIH, Z, coef) for Z, coef in atoms]
+    else:
+        out += [make_proc(PHOTO, Z, coef) for Z, coef in atoms]
+        out += [make_proc(BH, Z, coef) for Z, coef in atoms]
+        out += [make_proc(COMPTON, Z, coef) for Z, coef in atoms]
+    return out
+
+
+class OracleTable:
+    """collision_table_from_processes (src/collision_table.jl:115-149)."""
+
+    def __init__(self, procs, species, Fdt, safety=1.1, k=32, order=3, xmax=None,
+                 rate=None, ratebound=None, perm=None, extra_rb=None):
+        c = co()
+        self.k, self.order, self.species = k, order, species
+        self.xmax = xmax if xmax is not None else 1e3 * c.eV * 2 ** 18
+        n = len(procs)
+        if rate is None:
+            arr = (Proc * max(n, 1))(*procs)
+            rate = np.zeros((k + 1, n, order))
+            rb = np.zeros((k + 1, order))
+            perm = np.zeros(max(n, 1), dtype=np.int32)
+            keys = np.zeros(max(n, 1))
+            lib().or_build_table(arr, n, species, Fdt, safety, k, self.xmax, order, dptr(rate),
+                                 dptr(rb), perm.ctypes.data_as(C.POINTER(C.c_int)), dptr(keys))
+            self.keys = keys[:n]
+            ratebound = rb
+            perm = perm[:n]
+        self.perm = np.asarray(perm)
+        self.procs_unsorted = procs
+        self.procs = [procs[i] for i in self.perm]
+        self.rate = np.ascontiguousarray(rate, dtype=np.float64)
+        rb = np.ascontiguousarray(ratebound, dtype=np.float64).reshape(-1, k + 1, order)
+        if extra_rb is not None:
+            rb = np.concatenate([rb, np.asarray(extra_rb).reshape(-1, k + 1, order)])
+        self.ratebound = np.ascontiguousarray(rb)
+        self.nswarm = self.ratebound.shape[0]
+        self._arr = (Proc * max(n, 1))(*self.procs)
+        self.s = Table(k, order, n, self.nswarm, self.xmax, dptr(self.rate),
+                       dptr(self.ratebound), self._arr)
+
+    def select(self, eng, u, r):
+        part = np.zeros(len(self.procs) + 1)
+        j = lib().or_select(C.byref(self.s), eng, u, r, dptr(part))
+        return j, part
+
+
+def compratebound(rate_unsorted, species, Fdt, safety, k=32, order=3, xmax=None):
+    c = co()
+    xmax = xmax if xmax is not None else 1e3 * c.eV * 2 ** 18
+    a = np.ascontiguousarray(rate_unsorted)
+    rb = np.zeros((k + 1, order))
+    lib().or_compratebound(dptr(a), a.shape[1], species, Fdt, safety, k, xmax, order, dptr(rb))
+    return rb
+
+
+def precheb(x, k=32, xmax=None):
+    c = co()
+    xmax = xmax if xmax is not None else 1e3 * c.eV * 2 ** 18
+    t = np.zeros(3)
+    i = lib().or_precheb(x, k, xmax, dptr(t))
+    return i, t
+
+
+def field_homogeneous(e=(0.0, 0.0, 0.0), b=(0.0, 0.0, 0.0)):
+    f = Field()
+    f.e.kind = FIELD_HOMOGENEOUS
+    f.e.v[:] = list(e)
+    f.b.kind = FIELD_HOMOGENEOUS
+    f.b.v[:] = list(b)
+    return f
+
+
+def states_to_np(ptr, n):
+    buf = (State * n).from_address(C.addressof(ptr.contents)) if n else []
+    return np.frombuffer(buf, dtype=STATE_DTYPE, count=n).copy() if n else np.zeros(0, STATE_DTYPE)
+
+
+class OracleWorld:
+    """MultiPopulation + run loop of the reference (electron, photon, positron order)."""
+
+    def __init__(self, tables, seed, caps, cuts, tracked=(True, True, True), nthreads=1,
+                 field=None, fields=None, tally=None):
+        self.w = World()
+        lib().or_world_init(C.byref(self.w), seed, nthreads)
+        self.bufs = []
+        self.tables = tables
+        for s in range(3):
+            if tables[s] is None:
+                self.bufs.append(None)
+                continue
+            buf = np.zeros(max(caps[s], 1), dtype=STATE_DTYPE)
+            self.bufs.append(buf)
+            lib().or_pop_setup(C.byref(self.w), s, buf.ctypes.data_as(C.POINTER(State)),
+                               caps[s], C.byref(tables[s].s), cuts[s], int(tracked[s]))
+        self.hists = [None, None, None]
+        if tally is not None:
+            for s, (nb, lo, hi) in tally.items():
+                h = np.zeros(nb)
+                self.hists[s] = h
+                P = self.w.pop[s]
+                P.tally_hist = dptr(h)
+                P.tally_nbins, P.tally_lo, P.tally_hi = nb, lo, hi
+        if fields is not None:
+            for i, f in enumerate(fields):
+                lib().or_set_field(C.byref(self.w), i, C.byref(f))
+        else:
+            lib().or_set_field(C.byref(self.w), 0, C.byref(field or field_homogeneous()))
+
+    def add(self, species, states):
+        P = self.w.pop[species]
+        n0 = P.n
+        states = np.asarray(states, dtype=STATE_DTYPE)
+        assert n0 + states.size <= P.cap
+        self.bufs[species][n0:n0 + states.size] = states
+        P.n = n0 + states.size
+
+    def init(self):
+        """init! (src/mixed_population.jl:20-35)"""
+        for s in range(3):
+            if self.bufs[s] is None:
+                continue
+            for i in range(self.w.pop[s].n):
+                if self.bufs[s][i]["active"]:
+                    st = State.from_buffer(self.bufs[s], i * STATE_DTYPE.itemsize)
+                    lib().or_setr(C.byref(self.w), s, C.byref(st))
+
+    def advance(self, tfinal, droplow=True):
+        lib().or_advance(C.byref(self.w), tfinal)
+        if droplow:
+            for s in range(3):
+                if self.bufs[s] is not None:
+                    lib().or_droplow(C.byref(self.w), s)
+        self.w.step += 1
+
+    def run(self, t0, dt, nsteps, droplow=True):
+        t = t0
+        for _ in range(nsteps):
+            self.advance(t + dt, droplow)
+            t += dt
+        return t
+
+    def particles(self, species, active_only=True):
+        n = self.w.pop[species].n
+        a = self.bufs[species][:n].copy()
+        return a[a["active"] != 0] if active_only else a
+
+    def counts(self, species):
+        return np.array(self.w.counts[species][:])
+
+    def stats(self, species):
+        out = np.zeros(7)
+        lib().or_stats(C.byref(self.w.pop[species]), dptr(out))
+        return out
+
+
+def init_swarm(n, species, seed, uid0=0, swarm=0, e1=1e6, e2=None, loguniform=False):
+    """Initial particles (energies in eV): build-defined draw order, stream step 0xFFFFFFFF."""
+    eV = co().eV
+    out = np.zeros(n, dtype=STATE_DTYPE)
+    lib().or_init_swarm(out.ctypes.data_as(C.POINTER(State)), n, species, seed, uid0, swarm,
+                        e1 * eV, (e2 if e2 is not None else e1) * eV, int(loguniform))
+    return out

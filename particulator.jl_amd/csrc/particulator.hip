@@ -1,0 +1,1705 @@
+// particulator.hip -- kernels and C ABI of the null-collision Monte Carlo step on
+// MI355X (gfx950).
+//
+// Data layout in HBM (per species): ten FP64 columns x,y,z,px,py,pz,w,t,s,r, a u8 active
+// column, a u64 uid column and an i32 swarm column, each `capacity` long (structure of
+// arrays: every wave reads 64 consecutive doubles per field, fully coalesced).  A device
+// counter n holds the slots in use; secondaries are appended at n with an atomic claim.
+//
+// One step (pt_advance):
+//   k_snapshot   -- pass range [0, n) of every species (the advance_init!/iup state)
+//   k_advance    -- one thread per particle, grid-stride over all three species: ratebound,
+//                   then the reference's `while trem > eps` loop of push / null-collision
+//                   selection / sampling, appending secondaries; finally droplow!'s
+//                   energy cut.  Rate tables live in LDS.
+//   k_tail       -- one workgroup advances the secondaries created in this step (the
+//                   reference's further advance1! passes) until no new particle appears.
+// Compaction (pt_compact) fills holes below the active count with active particles from
+// above it (repack!), found by wave-ballot prefix sums over the active flags.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pt_device.h"
+
+using namespace pt;
+
+typedef unsigned long long ull;
+
+// ------------------------------------------------------------------ device parameters
+struct SpeciesDev {
+    double* f[PT_NFIELDS];
+    uint8_t* active;
+    uint64_t* uid;
+    int32_t* swarm;
+    ull* n;
+    ull cap;
+    int tracked;
+    int pad;
+};
+
+enum { FLAG_RATEBOUND = 0, FLAG_CAPACITY = 1, FLAG_RANGE = 2, FLAG_SAMPLER = 3, FLAG_TAIL = 4, NFLAGS = 5 };
+enum {
+    W_PSTEPS = 0, W_SECONDARIES = 1, W_MOVED = 2, W_EVENTS = 3, W_TAIL = 4, W_MAIN_SEC = 5,
+    NWORK = 8
+};
+constexpr int NCOUNT = MAX_PROCS + 1;
+
+struct Params {
+    SpeciesDev sp[PT_NSPECIES];
+    DevTable tab[PT_NSPECIES];
+    DevSB sb[MAX_SB];
+    DevPE pe[MAX_PE];
+    const Field* fields;
+    int nswarm;
+    int rb_in_lds;
+    uint64_t seed;
+    ull* counts;  // [3][NCOUNT]
+    ull* flags;   // [NFLAGS]
+    ull* work;    // [NWORK]
+    double* tally;  // [3][tally_bins + 2]
+    int tally_bins;
+    double tally_lo, tally_hi;
+    ull* range;  // [3][2]
+    int lds_rate_off[PT_NSPECIES];
+    int lds_rb_off[PT_NSPECIES];
+    int lds_doubles;
+};
+
+// ------------------------------------------------------------------ LDS tables
+struct Tabs {
+    const double* rate[PT_NSPECIES];
+    const double* rb[PT_NSPECIES];
+};
+
+__device__ void load_tables(const Params& P, double* lds, Tabs& T) {
+    for (int s = 0; s < PT_NSPECIES; s++) {
+        const DevTable& tb = P.tab[s];
+        if (!tb.valid) {
+            T.rate[s] = nullptr;
+            T.rb[s] = nullptr;
+            continue;
+        }
+        int nr = (tb.k + 1) * tb.nprocs * 3;
+        double* dst = lds + P.lds_rate_off[s];
+        for (int q = threadIdx.x; q < nr; q += blockDim.x) dst[q] = tb.rate[q];
+        T.rate[s] = dst;
+        if (P.rb_in_lds) {
+            int nb = (tb.k + 1) * 3;
+            double* d2 = lds + P.lds_rb_off[s];
+            for (int q = threadIdx.x; q < nb; q += blockDim.x) d2[q] = tb.ratebound[q];
+            T.rb[s] = d2;
+        } else {
+            T.rb[s] = tb.ratebound;
+        }
+    }
+    __syncthreads();
+}
+
+// setr!, src/collisions.jl:63-74
+__device__ __forceinline__ double setr(const Params& P, const Tabs& T, int sp, double E, int swarm,
+                                       bool& oob) {
+    const DevTable& tb = P.tab[sp];
+    if (E < tb.cut) return 0.0;
+    double t1, t2;
+    bool o;
+    int i = precheb(E, tb.k, tb.xmax, t1, t2, o);
+    oob |= o;
+    const double* rb = T.rb[sp] + (size_t)(tb.nswarm > 1 ? swarm : 0) * (tb.k + 1) * 3 + i * 3;
+    return cheb3(rb, t1, t2);
+}
+
+// block-local counters, flushed once per block
+struct BlockCounters {
+    ull counts[PT_NSPECIES * NCOUNT];
+    ull flags[NFLAGS];
+    ull work[NWORK];
+};
+
+__device__ void zero_counters(BlockCounters* bc) {
+    ull* p = (ull*)bc;
+    for (int q = threadIdx.x; q < (int)(sizeof(BlockCounters) / sizeof(ull)); q += blockDim.x) p[q] = 0;
+    __syncthreads();
+}
+
+__device__ void flush_counters(const Params& P, BlockCounters* bc) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < PT_NSPECIES * NCOUNT; q += blockDim.x)
+        if (bc->counts[q]) atomicAdd(&P.counts[q], bc->counts[q]);
+    for (int q = threadIdx.x; q < NFLAGS; q += blockDim.x)
+        if (bc->flags[q]) atomicAdd(&P.flags[q], bc->flags[q]);
+    for (int q = threadIdx.x; q < NWORK; q += blockDim.x)
+        if (bc->work[q]) atomicAdd(&P.work[q], bc->work[q]);
+}
+
+// ------------------------------------------------------------------ append
+// add_particle!, src/population.jl:103-113 (+ birth tallies of untracked species)
+__device__ void append(const Params& P, const Tabs& T, BlockCounters* bc, int sp2, const V3& x,
+                       const V3& p, double w, double t, double s, uint64_t uid, int swarm,
+                       bool main_pass) {
+    const DevTable& tb = P.tab[sp2];
+    double E = kinenergy(sp2, p);
+    if (E <= tb.cut) return;
+    const SpeciesDev& S = P.sp[sp2];
+    if (!S.tracked) {
+        if (P.tally) {
+            double* h = P.tally + (size_t)sp2 * (P.tally_bins + 2);
+            atomicAdd(&h[P.tally_bins], 1.0);
+            atomicAdd(&h[P.tally_bins + 1], w);
+            if (P.tally_bins > 0) {
+                double lx = log(E / P.tally_lo) / log(P.tally_hi / P.tally_lo);
+                int b = (int)floor(lx * P.tally_bins);
+                if (b >= 0 && b < P.tally_bins) atomicAdd(&h[b], w);
+            }
+        }
+        return;
+    }
+    if (!tb.valid) {
+        atomicAdd(&bc->flags[FLAG_CAPACITY], 1ull);
+        return;
+    }
+    ull slot = atomicAdd(S.n, 1ull);
+    if (slot >= S.cap) {
+        atomicAdd(&bc->flags[FLAG_CAPACITY], 1ull);
+        return;
+    }
+    bool oob = false;
+    double r = setr(P, T, sp2, E, swarm, oob);
+    if (oob) atomicAdd(&bc->flags[FLAG_RANGE], 1ull);
+    S.f[PT_X][slot] = x.x;
+    S.f[PT_Y][slot] = x.y;
+    S.f[PT_Z][slot] = x.z;
+    S.f[PT_PX][slot] = p.x;
+    S.f[PT_PY][slot] = p.y;
+    S.f[PT_PZ][slot] = p.z;
+    S.f[PT_W][slot] = w;
+    S.f[PT_T][slot] = t;
+    S.f[PT_S][slot] = s;
+    S.f[PT_R][slot] = r;
+    S.uid[slot] = uid;
+    S.swarm[slot] = swarm;
+    S.active[slot] = 1;
+    atomicAdd(&bc->work[W_SECONDARIES], 1ull);
+    if (main_pass) atomicAdd(&bc->work[W_MAIN_SEC], 1ull);
+}
+
+// ------------------------------------------------------------------ one particle
+// The body of advance1!'s @batch loop (src/mixed_population.jl:61-87) with
+// do_one_collision! (src/collisions.jl:142-199) and apply! (:83-133) inlined; r is set
+// first as advance_init! does, and droplow!'s cut is applied at the end.
+__device__ void advance_one(const Params& P, const Tabs& T, BlockCounters* bc, int sp, ull idx,
+                            double tfinal, uint32_t step, bool main_pass) {
+    const SpeciesDev& S = P.sp[sp];
+    if (!S.active[idx]) return;
+    double t = S.f[PT_T][idx];
+    double trem = tfinal - t;
+    if (!(trem > EPS64)) return;
+    V3 x{S.f[PT_X][idx], S.f[PT_Y][idx], S.f[PT_Z][idx]};
+    V3 p{S.f[PT_PX][idx], S.f[PT_PY][idx], S.f[PT_PZ][idx]};
+    double w = S.f[PT_W][idx];
+    double s = S.f[PT_S][idx];
+    uint64_t uid = S.uid[idx];
+    int swarm = S.swarm[idx];
+    if (swarm < 0 || swarm >= P.nswarm) swarm = 0;
+    const DevTable& tb = P.tab[sp];
+    const Field& F = P.fields[swarm];
+    bool oob = false;
+    double E = kinenergy(sp, p);
+    double r = setr(P, T, sp, E, swarm, oob);
+    PhiloxStream g;
+    g.init(P.seed, uid, step);
+    bool active = true;
+    ull nev = 0;
+    uint32_t iters = 0;
+    while (trem > EPS64 && active) {
+        if (++iters > ITER_LIMIT || g.exhausted()) {
+            atomicAdd(&bc->flags[FLAG_SAMPLER], 1ull);
+            break;
+        }
+        double tnext = s / r;
+        double dt;
+        bool coll;
+        if (trem > tnext) {
+            dt = tnext;
+            coll = true;
+        } else {
+            dt = trem;
+            coll = false;
+            s -= dt * r;
+        }
+        push(sp, F, x, p, t, dt);
+        if (coll && r != 0.0) {
+            E = kinenergy(sp, p);
+            if (E >= tb.cut) {
+                nev++;
+                double t1, t2;
+                bool o;
+                int iv = precheb(E, tb.k, tb.xmax, t1, t2, o);
+                oob |= o;
+                double xi = g.next() * r;
+                const double* rt = T.rate[sp] + (size_t)iv * tb.nprocs * 3;
+                int sel = -1;
+                for (int j = 0; j < tb.nprocs; j++) {
+                    double nu = cheb3(rt + 3 * j, t1, t2);
+                    if (nu > xi) {
+                        sel = j;
+                        break;
+                    }
+                    xi -= nu;
+                }
+                if (sel < 0) {
+                    if (!(xi >= 0.0)) atomicAdd(&bc->flags[FLAG_RATEBOUND], 1ull);
+                    atomicAdd(&bc->counts[sp * NCOUNT], 1ull);
+                    r = setr(P, T, sp, E, swarm, oob);
+                    s = nextcoll(g);
+                } else {
+                    atomicAdd(&bc->counts[sp * NCOUNT + 1 + sel], 1ull);
+                    Outcome oc;
+                    collide(tb.procs[sel], sp, p, E, P.sb, P.pe, g, oc);
+                    if (oc.err) atomicAdd(&bc->flags[FLAG_SAMPLER], 1ull);
+                    uint32_t dend = g.draw;
+                    if (oc.kind == OUT_STATECHANGE || oc.kind == OUT_NEW) {
+                        p = oc.p0;
+                        s = oc.s0;
+                        E = kinenergy(sp, p);
+                        r = setr(P, T, sp, E, swarm, oob);
+                    } else {
+                        active = false;
+                    }
+                    for (int q = 0; q < oc.nnew; q++)
+                        append(P, T, bc, oc.q[q].sp, x, oc.q[q].p, w, t, oc.q[q].s,
+                               child_uid(P.seed, uid, step, dend, q), swarm, main_pass);
+                }
+            }
+        }
+        trem -= dt;
+    }
+    if (active && kinenergy(sp, p) < tb.cut) active = false;  // droplow!
+    S.f[PT_X][idx] = x.x;
+    S.f[PT_Y][idx] = x.y;
+    S.f[PT_Z][idx] = x.z;
+    S.f[PT_PX][idx] = p.x;
+    S.f[PT_PY][idx] = p.y;
+    S.f[PT_PZ][idx] = p.z;
+    S.f[PT_T][idx] = t;
+    S.f[PT_S][idx] = s;
+    S.f[PT_R][idx] = r;
+    if (!active) S.active[idx] = 0;
+    if (oob) atomicAdd(&bc->flags[FLAG_RANGE], 1ull);
+    if (nev) atomicAdd(&bc->work[W_EVENTS], nev);
+    atomicAdd(&bc->work[main_pass ? W_PSTEPS : W_TAIL], 1ull);
+}
+
+// ------------------------------------------------------------------ step kernels
+__global__ void k_snapshot(const Params* __restrict__ Pp) {
+    const Params& P = *Pp;
+    int s = threadIdx.x;
+    if (s < PT_NSPECIES) {
+        ull n = *P.sp[s].n;
+        if (n > P.sp[s].cap) n = P.sp[s].cap;
+        P.range[2 * s] = 0;
+        P.range[2 * s + 1] = n;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_advance(const Params* __restrict__ Pp, double tfinal,
+                                                 uint32_t step) {
+    extern __shared__ double lds[];
+    __shared__ BlockCounters bc;
+    const Params& P = *Pp;
+    Tabs T;
+    zero_counters(&bc);
+    load_tables(P, lds, T);
+    ull b0 = P.range[0], e0 = P.range[1], b1 = P.range[2], e1 = P.range[3], b2 = P.range[4],
+        e2 = P.range[5];
+    ull n0 = e0 - b0, n1 = e1 - b1, n2 = e2 - b2;
+    ull total = n0 + n1 + n2;
+    ull stride = (ull)gridDim.x * blockDim.x;
+    for (ull q = (ull)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+        int sp;
+        ull idx;
+        if (q < n0) {
+            sp = 0;
+            idx = b0 + q;
+        } else if (q < n0 + n1) {
+            sp = 1;
+            idx = b1 + (q - n0);
+        } else {
+            sp = 2;
+            idx = b2 + (q - n0 - n1);
+        }
+        advance_one(P, T, &bc, sp, idx, tfinal, step, true);
+    }
+    flush_counters(P, &bc);
+}
+
+// Secondaries born in this step: one workgroup repeats passes over [begin, n) until a pass
+// appends nothing (the `while n > 0` loop of advance!, src/mixed_population.jl:43-46).
+__global__ void __launch_bounds__(1024) k_tail(const Params* __restrict__ Pp, double tfinal,
+                                               uint32_t step, int maxpasses) {
+    extern __shared__ double lds[];
+    __shared__ BlockCounters bc;
+    __shared__ ull beg[PT_NSPECIES], end[PT_NSPECIES];
+    __shared__ int done;
+    const Params& P = *Pp;
+    Tabs T;
+    zero_counters(&bc);
+    load_tables(P, lds, T);
+    if (threadIdx.x == 0)
+        for (int s = 0; s < PT_NSPECIES; s++) beg[s] = P.range[2 * s + 1];
+    for (int pass = 0; pass < maxpasses; pass++) {
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            done = 1;
+            for (int s = 0; s < PT_NSPECIES; s++) {
+                ull n = __hip_atomic_load(P.sp[s].n, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if (n > P.sp[s].cap) n = P.sp[s].cap;
+                end[s] = n;
+                if (n > beg[s]) done = 0;
+            }
+        }
+        __syncthreads();
+        if (done) break;
+        ull n0 = end[0] - beg[0], n1 = end[1] - beg[1], n2 = end[2] - beg[2];
+        ull total = n0 + n1 + n2;
+        for (ull q = threadIdx.x; q < total; q += blockDim.x) {
+            int sp;
+            ull idx;
+            if (q < n0) {
+                sp = 0;
+                idx = beg[0] + q;
+            } else if (q < n0 + n1) {
+                sp = 1;
+                idx = beg[1] + (q - n0);
+            } else {
+                sp = 2;
+                idx = beg[2] + (q - n0 - n1);
+            }
+            advance_one(P, T, &bc, sp, idx, tfinal, step, false);
+        }
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int s = 0; s < PT_NSPECIES; s++) beg[s] = end[s];
+    }
+    // the pass limit was reached with work left: report it
+    if (threadIdx.x == 0 && !done) bc.flags[FLAG_TAIL] += 1;
+    flush_counters(P, &bc);
+}
+
+// init! / advance_init!: r of every active particle (src/mixed_population.jl:20-35)
+__global__ void __launch_bounds__(256) k_setr(const Params* __restrict__ Pp, int sp) {
+    extern __shared__ double lds[];
+    const Params& P = *Pp;
+    Tabs T;
+    load_tables(P, lds, T);
+    const SpeciesDev& S = P.sp[sp];
+    ull n = *S.n;
+    if (n > S.cap) n = S.cap;
+    for (ull i = (ull)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ull)gridDim.x * blockDim.x) {
+        if (!S.active[i]) continue;
+        V3 p{S.f[PT_PX][i], S.f[PT_PY][i], S.f[PT_PZ][i]};
+        int sw = S.swarm[i];
+        if (sw < 0 || sw >= P.nswarm) sw = 0;
+        bool oob = false;
+        S.f[PT_R][i] = setr(P, T, sp, kinenergy(sp, p), sw, oob);
+        if (oob) atomicAdd(&P.flags[FLAG_RANGE], 1ull);
+    }
+}
+
+// initial particles generated on the device (same draws as the CPU restatement's
+// initialisation): energy, randsphere (src/util.jl:4-12), s = nextcoll()
+__global__ void __launch_bounds__(256) k_init_particles(const Params* __restrict__ Pp, int sp,
+                                                        ull base, ull n, double e1, double e2,
+                                                        int loguniform, int swarm, uint64_t uid0,
+                                                        double t0) {
+    const Params& P = *Pp;
+    const SpeciesDev& S = P.sp[sp];
+    for (ull i = (ull)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ull)gridDim.x * blockDim.x) {
+        ull slot = base + i;
+        uint64_t uid = uid0 + i;
+        PhiloxStream g;
+        g.init(P.seed, uid, 0xFFFFFFFFu);
+        double E = e1;
+        if (loguniform) {
+            double la = log(e1), lb = log(e2);
+            E = exp(la + g.next() * (lb - la));
+        }
+        V3 d = randsphere(g);
+        double pn = mom_from_kin(sp, E);
+        S.f[PT_X][slot] = 0.0;
+        S.f[PT_Y][slot] = 0.0;
+        S.f[PT_Z][slot] = 0.0;
+        S.f[PT_PX][slot] = pn * d.x;
+        S.f[PT_PY][slot] = pn * d.y;
+        S.f[PT_PZ][slot] = pn * d.z;
+        S.f[PT_W][slot] = 1.0;
+        S.f[PT_T][slot] = t0;
+        S.f[PT_S][slot] = nextcoll(g);
+        S.f[PT_R][slot] = 0.0;
+        S.uid[slot] = uid;
+        S.swarm[slot] = swarm;
+        S.active[slot] = 1;
+    }
+}
+
+// ------------------------------------------------------------------ compaction
+constexpr int CH = 4096;  // elements per chunk (one 256-thread block)
+
+__global__ void __launch_bounds__(256) k_cmp_count(const uint8_t* __restrict__ active, ull n,
+                                                   unsigned* __restrict__ cnt) {
+    __shared__ unsigned part[4];
+    ull c0 = (ull)blockIdx.x * CH;
+    unsigned local = 0;
+    for (int q = threadIdx.x; q < CH; q += blockDim.x) {
+        ull i = c0 + q;
+        if (i < n) local += active[i] ? 1u : 0u;
+    }
+    // wave reduction
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// exclusive scan of chunk counts; holes below A and tail actives at/above A per chunk.
+// info[0] = A (active count), info[1] = number of moves, info[2] = old n.
+__global__ void __launch_bounds__(1024) k_cmp_scan(const uint8_t* __restrict__ active, ull n,
+                                                   const unsigned* __restrict__ cnt, int nchunks,
+                                                   unsigned* __restrict__ hole_pre,
+                                                   unsigned* __restrict__ tail_pre,
+                                                   ull* __restrict__ info, ull* __restrict__ ncounter) {
+    __shared__ ull wsum[16];
+    __shared__ ull tot_s;
+    __shared__ unsigned act_low_s;
+    int tid = threadIdx.x;
+    int per = (nchunks + blockDim.x - 1) / blockDim.x;
+    int c0 = tid * per, c1 = min(nchunks, c0 + per);
+    // pass 1: total actives
+    ull mine = 0;
+    for (int c = c0; c < c1; c++) mine += cnt[c];
+    ull v = mine;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((tid & 63) == 0) wsum[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+        ull t = 0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += wsum[q];
+        tot_s = t;
+    }
+    __syncthreads();
+    ull A = tot_s;
+    int sc = (int)(A / CH);  // chunk containing index A
+    // actives in [sc*CH, A)
+    if (tid == 0) act_low_s = 0;
+    __syncthreads();
+    if (sc < nchunks) {
+        unsigned loc = 0;
+        for (ull i = (ull)sc * CH + tid; i < A; i += blockDim.x) loc += active[i] ? 1u : 0u;
+        if (loc) atomicAdd(&act_low_s, loc);
+    }
+    __syncthreads();
+    unsigned act_low = act_low_s;
+    // pass 2: per-thread sums of holes and tails
+    ull hs = 0, ts = 0;
+    for (int c = c0; c < c1; c++) {
+        unsigned h, tl;
+        if (c < sc) {
+            h = CH - cnt[c];
+            tl = 0;
+        } else if (c == sc) {
+            h = (unsigned)(A - (ull)sc * CH) - act_low;
+            tl = cnt[c] - act_low;
+        } else {
+            h = 0;
+            tl = cnt[c];
+        }
+        hs += h;
+        ts += tl;
+    }
+    // block exclusive scan of (hs, ts) in thread order
+    __shared__ ull sh[1024], st[1024];
+    sh[tid] = hs;
+    st[tid] = ts;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+        ull ah = tid >= off ? sh[tid - off] : 0, at = tid >= off ? st[tid - off] : 0;
+        __syncthreads();
+        sh[tid] += ah;
+        st[tid] += at;
+        __syncthreads();
+    }
+    ull hrun = sh[tid] - hs, trun = st[tid] - ts;
+    for (int c = c0; c < c1; c++) {
+        unsigned h, tl;
+        if (c < sc) {
+            h = CH - cnt[c];
+            tl = 0;
+        } else if (c == sc) {
+            h = (unsigned)(A - (ull)sc * CH) - act_low;
+            tl = cnt[c] - act_low;
+        } else {
+            h = 0;
+            tl = cnt[c];
+        }
+        hole_pre[c] = (unsigned)hrun;
+        tail_pre[c] = (unsigned)trun;
+        hrun += h;
+        trun += tl;
+    }
+    if (tid == (int)blockDim.x - 1) {
+        info[0] = A;
+        info[1] = sh[tid];  // total holes == total tail actives
+        info[2] = n;
+        *ncounter = A;
+    }
+}
+
+// positions of the k-th hole and the k-th tail active (ballot prefix sums, chunk order)
+__global__ void __launch_bounds__(256) k_cmp_fill(const uint8_t* __restrict__ active, ull n,
+                                                  const ull* __restrict__ info,
+                                                  const unsigned* __restrict__ hole_pre,
+                                                  const unsigned* __restrict__ tail_pre,
+                                                  unsigned* __restrict__ hpos,
+                                                  unsigned* __restrict__ tpos) {
+    __shared__ unsigned wh[4], wt[4];
+    __shared__ unsigned runh, runt;
+    ull A = info[0];
+    ull c0 = (ull)blockIdx.x * CH;
+    if (threadIdx.x == 0) {
+        runh = hole_pre[blockIdx.x];
+        runt = tail_pre[blockIdx.x];
+    }
+    __syncthreads();
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int base = 0; base < CH; base += 256) {
+        ull i = c0 + base + threadIdx.x;
+        bool act = i < n && active[i];
+        bool hole = i < A && !act;
+        bool tail = i >= A && i < n && act;
+        uint64_t bh = __ballot(hole), bt = __ballot(tail);
+        if (lane == 0) {
+            wh[wv] = __popcll(bh);
+            wt[wv] = __popcll(bt);
+        }
+        __syncthreads();
+        unsigned oh = runh, ot = runt;
+        for (int q = 0; q < wv; q++) {
+            oh += wh[q];
+            ot += wt[q];
+        }
+        if (hole) hpos[oh + __popcll(bh & lt)] = (unsigned)i;
+        if (tail) tpos[ot + __popcll(bt & lt)] = (unsigned)i;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            runh += wh[0] + wh[1] + wh[2] + wh[3];
+            runt += wt[0] + wt[1] + wt[2] + wt[3];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cmp_move(SpeciesDev S, const ull* __restrict__ info,
+                                                  const unsigned* __restrict__ hpos,
+                                                  const unsigned* __restrict__ tpos,
+                                                  ull* __restrict__ work) {
+    ull m = info[1];
+    for (ull k = (ull)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (ull)gridDim.x * blockDim.x) {
+        unsigned d = hpos[k], s = tpos[k];
+#pragma unroll
+        for (int f = 0; f < PT_NFIELDS; f++) S.f[f][d] = S.f[f][s];
+        S.uid[d] = S.uid[s];
+        S.swarm[d] = S.swarm[s];
+        S.active[d] = 1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&work[W_MOVED], m);
+}
+
+// ------------------------------------------------------------------ statistics
+// per-block partial sums over active particles: count, w, w*E, w*x, w*y, w*z, w*|x|^2, maxE
+constexpr int NSTAT = 8;
+__global__ void __launch_bounds__(256) k_stats_partial(const Params* __restrict__ Pp, int sp,
+                                                       double* __restrict__ part) {
+    const Params& P = *Pp;
+    const SpeciesDev& S = P.sp[sp];
+    ull n = *S.n;
+    if (n > S.cap) n = S.cap;
+    double a[NSTAT] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (ull i = (ull)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ull)gridDim.x * blockDim.x) {
+        if (!S.active[i]) continue;
+        V3 p{S.f[PT_PX][i], S.f[PT_PY][i], S.f[PT_PZ][i]};
+        V3 x{S.f[PT_X][i], S.f[PT_Y][i], S.f[PT_Z][i]};
+        double w = S.f[PT_W][i];
+        double E = kinenergy(sp, p);
+        a[0] += 1.0;
+        a[1] += w;
+        a[2] += w * E;
+        a[3] += w * x.x;
+        a[4] += w * x.y;
+        a[5] += w * x.z;
+        a[6] += w * dot3(x, x);
+        a[7] = fmax(a[7], E);
+    }
+    __shared__ double red[NSTAT][256];
+    for (int q = 0; q < NSTAT; q++) red[q][threadIdx.x] = a[q];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int q = 0; q < NSTAT; q++) {
+                if (q == 7) red[q][threadIdx.x] = fmax(red[q][threadIdx.x], red[q][threadIdx.x + off]);
+                else red[q][threadIdx.x] += red[q][threadIdx.x + off];
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < NSTAT) part[blockIdx.x * NSTAT + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ------------------------------------------------------------------ debug kernels
+__global__ void k_dbg_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out, ull n) {
+    ull i = (ull)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t a = ctr[4 * i], b = ctr[4 * i + 1], c = ctr[4 * i + 2], d = ctr[4 * i + 3];
+    philox10(a, b, c, d, k0, k1);
+    out[4 * i] = a;
+    out[4 * i + 1] = b;
+    out[4 * i + 2] = c;
+    out[4 * i + 3] = d;
+}
+
+__global__ void k_dbg_uniforms(uint64_t seed, uint64_t uid, uint32_t step, double* out, ull n) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    PhiloxStream g;
+    g.init(seed, uid, step);
+    for (ull i = 0; i < n; i++) out[i] = g.next();
+}
+
+__global__ void __launch_bounds__(256) k_dbg_select(const Params* __restrict__ Pp, int sp, ull n,
+                                                    const double* E, const double* u,
+                                                    const int32_t* swarm, int32_t* interval,
+                                                    int32_t* proc, double* rbound) {
+    extern __shared__ double lds[];
+    const Params& P = *Pp;
+    Tabs T;
+    load_tables(P, lds, T);
+    ull i = (ull)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevTable& tb = P.tab[sp];
+    int sw = swarm ? swarm[i] : 0;
+    if (sw < 0 || sw >= P.nswarm) sw = 0;
+    bool oob = false;
+    double r = setr(P, T, sp, E[i], sw, oob);
+    double t1, t2;
+    bool o;
+    int iv = precheb(E[i], tb.k, tb.xmax, t1, t2, o);
+    double xi = u[i] * r;
+    const double* rt = T.rate[sp] + (size_t)iv * tb.nprocs * 3;
+    int sel = -1;
+    for (int j = 0; j < tb.nprocs; j++) {
+        double nu = cheb3(rt + 3 * j, t1, t2);
+        if (nu > xi) {
+            sel = j;
+            break;
+        }
+        xi -= nu;
+    }
+    interval[i] = o ? -1 : iv;
+    proc[i] = sel;
+    rbound[i] = r;
+}
+
+__global__ void __launch_bounds__(256) k_dbg_push(const Params* __restrict__ Pp, int sp, ull n,
+                                                  double* f0, double* f1, double* f2, double* f3,
+                                                  double* f4, double* f5, double* f7,
+                                                  const double* dt) {
+    const Params& P = *Pp;
+    ull i = (ull)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 x{f0[i], f1[i], f2[i]}, p{f3[i], f4[i], f5[i]};
+    double t = f7[i];
+    push(sp, P.fields[0], x, p, t, dt[i]);
+    f0[i] = x.x;
+    f1[i] = x.y;
+    f2[i] = x.z;
+    f3[i] = p.x;
+    f4[i] = p.y;
+    f5[i] = p.z;
+    f7[i] = t;
+}
+
+__global__ void __launch_bounds__(256) k_dbg_collide(const Params* __restrict__ Pp, int sp, int proc,
+                                                     ull n, const double* in, const uint64_t* uid,
+                                                     const double* u, int stride, uint32_t step,
+                                                     int32_t* okind, int32_t* odraws, double* ost,
+                                                     int32_t* osp, uint64_t* ouid) {
+    const Params& P = *Pp;
+    ull i = (ull)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* st = in + i * PT_NFIELDS;
+    V3 x{st[0], st[1], st[2]}, p{st[3], st[4], st[5]};
+    double w = st[6], t = st[7];
+    InjectedStream g;
+    g.u = u + i * (ull)stride;
+    g.n = stride;
+    g.pos = 0;
+    g.draw = 0;
+    g.ran_out = false;
+    double E = kinenergy(sp, p);
+    Outcome oc;
+    collide(P.tab[sp].procs[proc], sp, p, E, P.sb, P.pe, g, oc);
+    okind[i] = g.ran_out ? -2 : (oc.err ? -1 - 16 * oc.err : oc.kind);
+    odraws[i] = (int32_t)g.draw;
+    double* o = ost + i * 3 * PT_NFIELDS;
+    for (int q = 0; q < 3 * PT_NFIELDS; q++) o[q] = 0.0;
+    for (int q = 0; q < 3; q++) {
+        osp[i * 3 + q] = -1;
+        ouid[i * 3 + q] = 0;
+    }
+    if (oc.kind == OUT_STATECHANGE || oc.kind == OUT_NEW) {
+        double v[PT_NFIELDS] = {x.x, x.y, x.z, oc.p0.x, oc.p0.y, oc.p0.z, w, t, oc.s0, 0.0};
+        for (int f = 0; f < PT_NFIELDS; f++) o[f] = v[f];
+        osp[i * 3] = sp;
+        ouid[i * 3] = uid[i];
+    }
+    for (int q = 0; q < oc.nnew; q++) {
+        double v[PT_NFIELDS] = {x.x, x.y, x.z, oc.q[q].p.x, oc.q[q].p.y, oc.q[q].p.z, w, t, oc.q[q].s, 0.0};
+        for (int f = 0; f < PT_NFIELDS; f++) o[(1 + q) * PT_NFIELDS + f] = v[f];
+        osp[i * 3 + 1 + q] = oc.q[q].sp;
+        ouid[i * 3 + 1 + q] = child_uid(P.seed, uid[i], step, g.draw, q);
+    }
+}
+
+// ================================================================== host side
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct pt_ctx {
+    pt_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Params hp{};          // host copy of the parameters
+    Params* dp = nullptr;  // device copy
+    bool dirty = true;
+    std::vector<void*> allocs;
+    // per species
+    ull* dn = nullptr;  // [3] counters
+    ull hn[PT_NSPECIES] = {0, 0, 0};  // host view (valid after sync)
+    // tables
+    double* drate[PT_NSPECIES] = {nullptr, nullptr, nullptr};
+    double* drb[PT_NSPECIES] = {nullptr, nullptr, nullptr};
+    int L[PT_NSPECIES] = {0, 0, 0};
+    int kint[PT_NSPECIES] = {0, 0, 0};
+    double* dsb_le[MAX_SB] = {};
+    double* dsb_data[MAX_SB] = {};
+    Field* dfields = nullptr;
+    std::vector<Field> hfields;
+    // compaction scratch
+    unsigned* ccnt = nullptr;
+    unsigned* chole = nullptr;
+    unsigned* ctail = nullptr;
+    unsigned* hpos = nullptr;
+    unsigned* tpos = nullptr;
+    ull* cinfo = nullptr;
+    size_t cchunks = 0, cposcap = 0;
+    double* stat_part = nullptr;
+    // counters
+    ull* dcounts = nullptr;
+    ull* dflags = nullptr;
+    ull* dwork = nullptr;
+    ull* drange = nullptr;
+    double* dtally = nullptr;
+    uint32_t step = 0;
+    uint64_t next_uid = 0;
+    int grid = 0;
+    int lds_bytes = 0;
+    bool profile = false;
+    hipEvent_t ev[8] = {};
+    std::vector<hipEvent_t> evq;  // (begin, middle, end) per profiled step, read lazily
+    size_t evq_used = 0;
+    double adv_ms = 0.0, tail_ms = 0.0, cmp_ms = 0.0;
+    ull adv_launches = 0;
+    std::string err;
+};
+
+static int fail(pt_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                     \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((c), PT_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+static int dalloc(pt_ctx* c, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(c, PT_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    c->allocs.push_back(*p);
+    return PT_OK;
+}
+
+static int sync_params(pt_ctx* c) {
+    if (!c->dirty) return PT_OK;
+    // LDS layout of the rate tables
+    int off = 0;
+    c->hp.rb_in_lds = c->cfg.nswarm <= 1 ? 1 : 0;
+    for (int s = 0; s < PT_NSPECIES; s++) {
+        c->hp.lds_rate_off[s] = off;
+        if (c->hp.tab[s].valid) off += (c->hp.tab[s].k + 1) * c->hp.tab[s].nprocs * 3;
+        c->hp.lds_rb_off[s] = off;
+        if (c->hp.tab[s].valid && c->hp.rb_in_lds) off += (c->hp.tab[s].k + 1) * 3;
+    }
+    c->hp.lds_doubles = off;
+    c->lds_bytes = off * (int)sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(c->dp, &c->hp, sizeof(Params), hipMemcpyHostToDevice, c->stream));
+    c->dirty = false;
+    return PT_OK;
+}
+
+static int read_counts(pt_ctx* c) {
+    HIPCHK(c, hipMemcpyAsync(c->hn, c->dn, sizeof(ull) * PT_NSPECIES, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int s = 0; s < PT_NSPECIES; s++)
+        if (c->hn[s] > c->cfg.capacity[s]) c->hn[s] = c->cfg.capacity[s];
+    return PT_OK;
+}
+
+template <class T>
+struct Scratch {
+    T* p = nullptr;
+    ~Scratch() {
+        if (p) hipFree(p);
+    }
+};
+
+extern "C" {
+
+int pt_abi_version(void) { return PT_ABI_VERSION; }
+
+const char* pt_last_error(const pt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void pt_destroy(pt_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) hipFree(p);
+    for (auto& e : c->ev)
+        if (e) hipEventDestroy(e);
+    for (auto& e : c->evq)
+        if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int pt_create(const pt_config* cfg, pt_ctx** out) {
+    if (!cfg || !out) return PT_E_ARG;
+    *out = nullptr;
+    pt_ctx* c = new (std::nothrow) pt_ctx();
+    if (!c) return PT_E_NOMEM;
+    c->cfg = *cfg;
+    if (c->cfg.nswarm < 1) c->cfg.nswarm = 1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        int rc = fail(c, PT_E_NOT_BUILT, "no HIP device available");
+        pt_destroy(c);
+        return rc;
+    }
+    for (int s = 0; s < PT_NSPECIES; s++) {
+        if (c->cfg.capacity[s] >= (1ull << 32)) {
+            int rc = fail(c, PT_E_ARG, "capacity of species %d exceeds 2^32 - 1", s);
+            pt_destroy(c);
+            return rc;
+        }
+    }
+    c->device = cfg->device;
+    int rc = PT_OK;
+#define TRY(x)             \
+    do {                   \
+        rc = (x);          \
+        if (rc != PT_OK) { \
+            *out = c;      \
+            return rc;     \
+        }                  \
+    } while (0)
+    if (hipSetDevice(c->device) != hipSuccess) TRY(fail(c, PT_E_HIP, "hipSetDevice(%d) failed", c->device));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        TRY(fail(c, PT_E_HIP, "hipStreamCreate failed"));
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) TRY(fail(c, PT_E_HIP, "hipEventCreate failed"));
+    Params& P = c->hp;
+    memset(&P, 0, sizeof(P));
+    TRY(dalloc(c, (void**)&c->dn, sizeof(ull) * PT_NSPECIES));
+    size_t maxcap = 0;
+    for (int s = 0; s < PT_NSPECIES; s++) {
+        SpeciesDev& S = P.sp[s];
+        ull cap = c->cfg.capacity[s];
+        maxcap = std::max<size_t>(maxcap, cap);
+        S.cap = cap;
+        S.tracked = c->cfg.tracked[s];
+        S.n = c->dn + s;
+        for (int f = 0; f < PT_NFIELDS; f++) TRY(dalloc(c, (void**)&S.f[f], sizeof(double) * cap));
+        TRY(dalloc(c, (void**)&S.active, cap));
+        TRY(dalloc(c, (void**)&S.uid, sizeof(uint64_t) * cap));
+        TRY(dalloc(c, (void**)&S.swarm, sizeof(int32_t) * cap));
+        HIPCHK(c, hipMemsetAsync(S.active, 0, cap ? cap : 1, c->stream));
+    }
+    HIPCHK(c, hipMemsetAsync(c->dn, 0, sizeof(ull) * PT_NSPECIES, c->stream));
+    TRY(dalloc(c, (void**)&c->dcounts, sizeof(ull) * PT_NSPECIES * NCOUNT));
+    TRY(dalloc(c, (void**)&c->dflags, sizeof(ull) * NFLAGS));
+    TRY(dalloc(c, (void**)&c->dwork, sizeof(ull) * NWORK));
+    TRY(dalloc(c, (void**)&c->drange, sizeof(ull) * 2 * PT_NSPECIES));
+    HIPCHK(c, hipMemsetAsync(c->dcounts, 0, sizeof(ull) * PT_NSPECIES * NCOUNT, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->dflags, 0, sizeof(ull) * NFLAGS, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->dwork, 0, sizeof(ull) * NWORK, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->drange, 0, sizeof(ull) * 2 * PT_NSPECIES, c->stream));
+    int bins = std::max(0, c->cfg.tally_bins);
+    TRY(dalloc(c, (void**)&c->dtally, sizeof(double) * PT_NSPECIES * (bins + 2)));
+    HIPCHK(c, hipMemsetAsync(c->dtally, 0, sizeof(double) * PT_NSPECIES * (bins + 2), c->stream));
+    P.tally = c->dtally;
+    P.tally_bins = bins;
+    P.tally_lo = c->cfg.tally_lo > 0 ? c->cfg.tally_lo : 1e3 * EV;
+    P.tally_hi = c->cfg.tally_hi > P.tally_lo ? c->cfg.tally_hi : 1e7 * EV;
+    // fields
+    c->hfields.assign(c->cfg.nswarm, Field{});
+    TRY(dalloc(c, (void**)&c->dfields, sizeof(Field) * c->cfg.nswarm));
+    HIPCHK(c, hipMemcpyAsync(c->dfields, c->hfields.data(), sizeof(Field) * c->cfg.nswarm,
+                             hipMemcpyHostToDevice, c->stream));
+    P.fields = c->dfields;
+    P.nswarm = c->cfg.nswarm;
+    P.seed = c->cfg.seed;
+    P.counts = c->dcounts;
+    P.flags = c->dflags;
+    P.work = c->dwork;
+    P.range = c->drange;
+    // compaction scratch
+    c->cchunks = (maxcap + CH - 1) / CH + 1;
+    TRY(dalloc(c, (void**)&c->ccnt, sizeof(unsigned) * c->cchunks));
+    TRY(dalloc(c, (void**)&c->chole, sizeof(unsigned) * c->cchunks));
+    TRY(dalloc(c, (void**)&c->ctail, sizeof(unsigned) * c->cchunks));
+    c->cposcap = maxcap / 2 + 1;
+    TRY(dalloc(c, (void**)&c->hpos, sizeof(unsigned) * c->cposcap));
+    TRY(dalloc(c, (void**)&c->tpos, sizeof(unsigned) * c->cposcap));
+    TRY(dalloc(c, (void**)&c->cinfo, sizeof(ull) * 4));
+    TRY(dalloc(c, (void**)&c->stat_part, sizeof(double) * NSTAT * 4096));
+    TRY(dalloc(c, (void**)&c->dp, sizeof(Params)));
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    c->grid = prop.multiProcessorCount * 8;
+    c->dirty = true;
+    TRY(sync_params(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+#undef TRY
+    *out = c;
+    return PT_OK;
+}
+
+int pt_upload_table(pt_ctx* c, int sp, const pt_table_desc* t) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES || !t) return fail(c, PT_E_ARG, "bad species or table");
+    if (t->order != 3) return fail(c, PT_E_ARG, "Chebyshev order must be 3 (got %d)", t->order);
+    if (t->nprocs < 0 || t->nprocs > MAX_PROCS) return fail(c, PT_E_ARG, "nprocs %d out of [0, %d]", t->nprocs, MAX_PROCS);
+    if (t->nintervals < 1 || t->nintervals >= MAX_INTERVALS) return fail(c, PT_E_ARG, "nintervals out of range");
+    int nsw = t->nswarm < 1 ? 1 : t->nswarm;
+    if (nsw != 1 && nsw != c->cfg.nswarm) return fail(c, PT_E_ARG, "table nswarm %d != context nswarm %d", nsw, c->cfg.nswarm);
+    if (!t->rate || !t->ratebound || (t->nprocs > 0 && !t->procs)) return fail(c, PT_E_ARG, "null table arrays");
+    HIPCHK(c, hipSetDevice(c->device));
+    int k = t->nintervals;
+    size_t nr = (size_t)(k + 1) * t->nprocs * 3, nb = (size_t)(k + 1) * 3 * c->cfg.nswarm;
+    if (!c->drate[sp] || c->L[sp] != t->nprocs || c->kint[sp] != k) {
+        int rc = dalloc(c, (void**)&c->drate[sp], sizeof(double) * (nr ? nr : 1));
+        if (rc) return rc;
+        rc = dalloc(c, (void**)&c->drb[sp], sizeof(double) * nb);
+        if (rc) return rc;
+        c->L[sp] = t->nprocs;
+        c->kint[sp] = k;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->drate[sp], t->rate, sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+    if (nsw == 1) {
+        for (int q = 0; q < c->cfg.nswarm; q++)
+            HIPCHK(c, hipMemcpyAsync(c->drb[sp] + (size_t)q * (k + 1) * 3, t->ratebound,
+                                     sizeof(double) * (k + 1) * 3, hipMemcpyHostToDevice, c->stream));
+    } else {
+        HIPCHK(c, hipMemcpyAsync(c->drb[sp], t->ratebound, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
+    }
+    DevTable& D = c->hp.tab[sp];
+    D.nprocs = t->nprocs;
+    D.k = k;
+    D.nswarm = c->cfg.nswarm;
+    D.valid = 1;
+    D.xmax = t->xmax;
+    D.cut = t->energy_cut;
+    D.rate = c->drate[sp];
+    D.ratebound = c->drb[sp];
+    for (int j = 0; j < t->nprocs; j++) {
+        const pt_proc_desc& s = t->procs[j];
+        DevProc& d = D.procs[j];
+        d.kind = s.kind;
+        d.Z = s.Z;
+        d.slot = s.slot;
+        d.N = s.N;
+        d.B = s.B;
+        d.U = s.U;
+        d.tcut = s.tcut;
+        d.zf = s.zf;
+        for (int q = 0; q < 4; q++) d.c[q] = s.c[q];
+        if (s.kind < PT_PROC_ELASTIC || s.kind > PT_PROC_KN_COMPTON)
+            return fail(c, PT_E_ARG, "process %d: unknown kind %d", j, s.kind);
+        if ((s.kind == PT_PROC_SB && (s.slot < 0 || s.slot >= MAX_SB)) ||
+            (s.kind == PT_PROC_PHOTOELECTRIC && (s.slot < 0 || s.slot >= MAX_PE)))
+            return fail(c, PT_E_ARG, "process %d: data slot %d out of range", j, s.slot);
+        if (s.kind == PT_PROC_RBEB && !(s.B > 0)) return fail(c, PT_E_ARG, "process %d: RBEB needs B > 0", j);
+        if (s.kind == PT_PROC_ELASTIC && !(s.c[0] > 0))
+            return fail(c, PT_E_ARG, "process %d: elastic needs the screening radius c[0]", j);
+        if (s.kind == PT_PROC_BETHE_HEITLER && !(s.c[0] > 0))
+            return fail(c, PT_E_ARG, "process %d: Bethe-Heitler needs c[0] = Z^(1/3)", j);
+    }
+    c->dirty = true;
+    int rc = sync_params(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+int pt_upload_sb(pt_ctx* c, int slot, int Z, int ne, int ncum, const double* le, const double* data) {
+    if (!c) return PT_E_ARG;
+    if (slot < 0 || slot >= MAX_SB || ne < 2 || ncum < 2 || !le || !data)
+        return fail(c, PT_E_ARG, "bad SB upload arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = dalloc(c, (void**)&c->dsb_le[slot], sizeof(double) * ne);
+    if (rc) return rc;
+    rc = dalloc(c, (void**)&c->dsb_data[slot], sizeof(double) * (size_t)ne * ncum);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->dsb_le[slot], le, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->dsb_data[slot], data, sizeof(double) * (size_t)ne * ncum, hipMemcpyHostToDevice, c->stream));
+    c->hp.sb[slot].ne = ne;
+    c->hp.sb[slot].ncum = ncum;
+    c->hp.sb[slot].log_energy = c->dsb_le[slot];
+    c->hp.sb[slot].data = c->dsb_data[slot];
+    (void)Z;
+    c->dirty = true;
+    rc = sync_params(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+int pt_upload_pe(pt_ctx* c, int slot, int Z, int nshell, const double* binding) {
+    if (!c) return PT_E_ARG;
+    if (slot < 0 || slot >= MAX_PE || nshell < 1 || nshell > MAX_SHELLS || !binding)
+        return fail(c, PT_E_ARG, "bad photoelectric upload arguments");
+    c->hp.pe[slot].nshell = nshell;
+    for (int q = 0; q < nshell; q++) c->hp.pe[slot].binding[q] = binding[q];
+    (void)Z;
+    c->dirty = true;
+    return sync_params(c);
+}
+
+int pt_set_field(pt_ctx* c, int swarm, const pt_field_desc* f) {
+    if (!c) return PT_E_ARG;
+    if (swarm < 0 || swarm >= c->cfg.nswarm || !f) return fail(c, PT_E_ARG, "bad swarm %d", swarm);
+    Field& F = c->hfields[swarm];
+    const pt_field_fn* src[2] = {&f->e, &f->b};
+    FieldFn* dst[2] = {&F.e, &F.b};
+    for (int q = 0; q < 2; q++) {
+        dst[q]->kind = src[q]->kind;
+        for (int d = 0; d < 3; d++) {
+            dst[q]->v[d] = src[q]->v[d];
+            dst[q]->v2[d] = src[q]->v2[d];
+        }
+        dst[q]->z1 = src[q]->z1;
+        dst[q]->z2 = src[q]->z2;
+        dst[q]->sx = src[q]->sx;
+        dst[q]->sy = src[q]->sy;
+        dst[q]->sz = src[q]->sz;
+        dst[q]->ez0 = src[q]->ez0;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->dfields + swarm, &F, sizeof(Field), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+int pt_count(pt_ctx* c, int sp, uint64_t* n) {
+    if (!c || sp < 0 || sp >= PT_NSPECIES || !n) return c ? fail(c, PT_E_ARG, "bad arguments") : PT_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = read_counts(c);
+    if (rc) return rc;
+    *n = c->hn[sp];
+    return PT_OK;
+}
+
+static int put_particles(pt_ctx* c, int sp, ull base, const double* const soa[PT_NFIELDS],
+                         const uint8_t* active, const uint64_t* uid, const int32_t* swarm, size_t n) {
+    SpeciesDev& S = c->hp.sp[sp];
+    for (int f = 0; f < PT_NFIELDS; f++) {
+        if (soa[f]) {
+            HIPCHK(c, hipMemcpyAsync(S.f[f] + base, soa[f], sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        } else {
+            HIPCHK(c, hipMemsetAsync(S.f[f] + base, 0, sizeof(double) * n, c->stream));
+        }
+    }
+    std::vector<uint64_t> u;
+    if (!uid) {
+        u.resize(n);
+        for (size_t i = 0; i < n; i++) u[i] = c->next_uid + i;
+        c->next_uid += n;
+        uid = u.data();
+    }
+    std::vector<int32_t> sw;
+    if (!swarm) {
+        sw.assign(n, 0);
+        swarm = sw.data();
+    }
+    for (size_t i = 0; i < n; i++)
+        if (swarm[i] < 0 || swarm[i] >= c->cfg.nswarm)
+            return fail(c, PT_E_ARG, "particle %zu: swarm %d out of range", i, swarm[i]);
+    std::vector<uint8_t> act;
+    if (!active) {
+        act.assign(n, 1);
+        active = act.data();
+    }
+    HIPCHK(c, hipMemcpyAsync(S.uid + base, uid, sizeof(uint64_t) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(S.swarm + base, swarm, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(S.active + base, active, n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+int pt_add_particles(pt_ctx* c, int sp, const double* const soa[PT_NFIELDS], const uint64_t* uid,
+                     const int32_t* swarm, size_t n) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES || !soa) return fail(c, PT_E_ARG, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = read_counts(c);
+    if (rc) return rc;
+    if (c->hn[sp] + n > c->cfg.capacity[sp])
+        return fail(c, PT_E_CAPACITY, "species %d: %llu + %zu particles exceed capacity %llu", sp,
+                    (unsigned long long)c->hn[sp], n, (unsigned long long)c->cfg.capacity[sp]);
+    rc = put_particles(c, sp, c->hn[sp], soa, nullptr, uid, swarm, n);
+    if (rc) return rc;
+    ull nn = c->hn[sp] + n;
+    HIPCHK(c, hipMemcpyAsync(c->dn + sp, &nn, sizeof(ull), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->hn[sp] = nn;
+    return PT_OK;
+}
+
+int pt_set_particles(pt_ctx* c, int sp, const double* const soa[PT_NFIELDS], const uint8_t* active,
+                     const uint64_t* uid, const int32_t* swarm, size_t n) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES || !soa) return fail(c, PT_E_ARG, "bad arguments");
+    if (n > c->cfg.capacity[sp]) return fail(c, PT_E_CAPACITY, "%zu particles exceed capacity", n);
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = put_particles(c, sp, 0, soa, active, uid, swarm, n);
+    if (rc) return rc;
+    ull nn = n;
+    HIPCHK(c, hipMemcpyAsync(c->dn + sp, &nn, sizeof(ull), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->hn[sp] = nn;
+    return PT_OK;
+}
+
+int pt_init_particles(pt_ctx* c, int sp, size_t n, double e1, double e2, int loguniform, int swarm,
+                      uint64_t uid0, double t0) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES) return fail(c, PT_E_ARG, "bad species");
+    if (swarm < 0 || swarm >= c->cfg.nswarm) return fail(c, PT_E_ARG, "bad swarm");
+    if (!(e1 > 0) || (loguniform && !(e2 > 0))) return fail(c, PT_E_ARG, "energies must be positive");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = read_counts(c);
+    if (rc) return rc;
+    if (c->hn[sp] + n > c->cfg.capacity[sp]) return fail(c, PT_E_CAPACITY, "initial particles exceed capacity");
+    rc = sync_params(c);
+    if (rc) return rc;
+    if (n) {
+        int blocks = (int)std::min<size_t>((n + 255) / 256, 65535);
+        hipLaunchKernelGGL(k_init_particles, dim3(blocks), dim3(256), 0, c->stream, c->dp, sp,
+                           (ull)c->hn[sp], (ull)n, e1, e2, loguniform, swarm, uid0, t0);
+        HIPCHK(c, hipGetLastError());
+    }
+    ull nn = c->hn[sp] + n;
+    HIPCHK(c, hipMemcpyAsync(c->dn + sp, &nn, sizeof(ull), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->hn[sp] = nn;
+    if (uid0 + n > c->next_uid) c->next_uid = uid0 + n;
+    return PT_OK;
+}
+
+int pt_download(pt_ctx* c, int sp, double* const soa[PT_NFIELDS], uint8_t* active, uint64_t* uid,
+                int32_t* swarm, size_t cap, size_t* n) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES) return fail(c, PT_E_ARG, "bad species");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = read_counts(c);
+    if (rc) return rc;
+    size_t m = c->hn[sp];
+    if (n) *n = m;
+    if (m > cap) return fail(c, PT_E_CAPACITY, "host buffer of %zu too small for %zu particles", cap, m);
+    SpeciesDev& S = c->hp.sp[sp];
+    if (m) {
+        for (int f = 0; f < PT_NFIELDS; f++)
+            if (soa && soa[f]) HIPCHK(c, hipMemcpyAsync(soa[f], S.f[f], sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+        if (active) HIPCHK(c, hipMemcpyAsync(active, S.active, m, hipMemcpyDeviceToHost, c->stream));
+        if (uid) HIPCHK(c, hipMemcpyAsync(uid, S.uid, sizeof(uint64_t) * m, hipMemcpyDeviceToHost, c->stream));
+        if (swarm) HIPCHK(c, hipMemcpyAsync(swarm, S.swarm, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+static int require_tables(pt_ctx* c) {
+    for (int s = 0; s < PT_NSPECIES; s++) {
+        if (c->cfg.tracked[s] && !c->hp.tab[s].valid && c->cfg.capacity[s] > 0)
+            return fail(c, PT_E_STATE, "species %d is tracked but has no collision table", s);
+        const DevTable& tb = c->hp.tab[s];
+        if (!tb.valid) continue;
+        for (int j = 0; j < tb.nprocs; j++) {
+            const DevProc& d = tb.procs[j];
+            if (d.kind == PT_PROC_SB && (!c->hp.sb[d.slot].data || c->hp.sb[d.slot].ne < 2))
+                return fail(c, PT_E_STATE, "species %d process %d: SB slot %d not uploaded", s, j, d.slot);
+            if (d.kind == PT_PROC_PHOTOELECTRIC && c->hp.pe[d.slot].nshell < 1)
+                return fail(c, PT_E_STATE, "species %d process %d: photoelectric slot %d not uploaded", s, j, d.slot);
+        }
+    }
+    return PT_OK;
+}
+
+int pt_init(pt_ctx* c) {
+    if (!c) return PT_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = require_tables(c);
+    if (rc) return rc;
+    rc = sync_params(c);
+    if (rc) return rc;
+    for (int s = 0; s < PT_NSPECIES; s++) {
+        if (!c->hp.tab[s].valid) continue;
+        hipLaunchKernelGGL(k_setr, dim3(c->grid), dim3(256), c->lds_bytes, c->stream, c->dp, s);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+// accumulate the durations of the profiled steps recorded so far
+static int drain_events(pt_ctx* c) {
+    if (c->evq_used == 0) return PT_OK;
+    HIPCHK(c, hipEventSynchronize(c->evq[c->evq_used - 1]));
+    for (size_t q = 0; q + 3 <= c->evq_used; q += 3) {
+        float a = 0, b = 0;
+        HIPCHK(c, hipEventElapsedTime(&a, c->evq[q], c->evq[q + 1]));
+        HIPCHK(c, hipEventElapsedTime(&b, c->evq[q + 1], c->evq[q + 2]));
+        c->adv_ms += a;
+        c->tail_ms += b;
+        c->adv_launches++;
+    }
+    c->evq_used = 0;
+    return PT_OK;
+}
+
+static int next_event(pt_ctx* c, hipEvent_t* e) {
+    if (c->evq_used == c->evq.size()) {
+        hipEvent_t ne;
+        HIPCHK(c, hipEventCreate(&ne));
+        c->evq.push_back(ne);
+    }
+    *e = c->evq[c->evq_used++];
+    return PT_OK;
+}
+
+static int launch_step(pt_ctx* c, double tfinal) {
+    hipLaunchKernelGGL(k_snapshot, dim3(1), dim3(64), 0, c->stream, c->dp);
+    HIPCHK(c, hipGetLastError());
+    ull est = c->hn[0] + c->hn[1] + c->hn[2];
+    int blocks = (int)std::min<ull>((ull)c->grid, std::max<ull>(1, (est + 255) / 256));
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    int rc;
+    if (c->profile) {
+        if (c->evq_used + 3 > 3 * 1024 && (rc = drain_events(c))) return rc;
+        if ((rc = next_event(c, &e0)) || (rc = next_event(c, &e1)) || (rc = next_event(c, &e2))) return rc;
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+    }
+    hipLaunchKernelGGL(k_advance, dim3(blocks), dim3(256), c->lds_bytes, c->stream, c->dp, tfinal, c->step);
+    HIPCHK(c, hipGetLastError());
+    if (c->profile) HIPCHK(c, hipEventRecord(e1, c->stream));
+    hipLaunchKernelGGL(k_tail, dim3(1), dim3(1024), c->lds_bytes, c->stream, c->dp, tfinal, c->step, 100000);
+    HIPCHK(c, hipGetLastError());
+    if (c->profile) HIPCHK(c, hipEventRecord(e2, c->stream));
+    c->step++;
+    return PT_OK;
+}
+
+static int check_flags(pt_ctx* c, int64_t* out) {
+    ull f[NFLAGS];
+    HIPCHK(c, hipMemcpyAsync(f, c->dflags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (out)
+        for (int q = 0; q < NFLAGS; q++) out[q] = (int64_t)f[q];
+    if (f[FLAG_CAPACITY]) return fail(c, PT_E_CAPACITY, "%llu particles did not fit in their population", f[FLAG_CAPACITY]);
+    if (f[FLAG_SAMPLER]) return fail(c, PT_E_SAMPLER, "%llu sampler domain failures", f[FLAG_SAMPLER]);
+    if (f[FLAG_RANGE]) return fail(c, PT_E_TABLE_RANGE, "%llu energies above the table range", f[FLAG_RANGE]);
+    if (f[FLAG_TAIL]) return fail(c, PT_E_STATE, "secondary passes did not finish within the pass limit");
+    if (f[FLAG_RATEBOUND]) return fail(c, PT_E_RATEBOUND, "rate bound underestimated %llu times: increase safety factor", f[FLAG_RATEBOUND]);
+    return PT_OK;
+}
+
+int pt_check(pt_ctx* c, int64_t* flags) {
+    if (!c) return PT_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    return check_flags(c, flags);
+}
+
+int pt_advance(pt_ctx* c, double tfinal) {
+    if (!c) return PT_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = require_tables(c);
+    if (rc) return rc;
+    rc = sync_params(c);
+    if (rc) return rc;
+    rc = launch_step(c, tfinal);
+    if (rc) return rc;
+    rc = read_counts(c);
+    if (rc) return rc;
+    return check_flags(c, nullptr);
+}
+
+static int compact_species(pt_ctx* c, int sp) {
+    ull n = c->hn[sp];
+    if (n == 0) return PT_OK;
+    int nch = (int)((n + CH - 1) / CH);
+    SpeciesDev& S = c->hp.sp[sp];
+    hipLaunchKernelGGL(k_cmp_count, dim3(nch), dim3(256), 0, c->stream, S.active, n, c->ccnt);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_cmp_scan, dim3(1), dim3(1024), 0, c->stream, S.active, n, c->ccnt, nch, c->chole,
+                       c->ctail, c->cinfo, c->dn + sp);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_cmp_fill, dim3(nch), dim3(256), 0, c->stream, S.active, n, c->cinfo, c->chole,
+                       c->ctail, c->hpos, c->tpos);
+    HIPCHK(c, hipGetLastError());
+    int mb = (int)std::min<ull>((ull)c->grid, std::max<ull>(1, (n / 2 + 255) / 256));
+    hipLaunchKernelGGL(k_cmp_move, dim3(mb), dim3(256), 0, c->stream, S, c->cinfo, c->hpos, c->tpos, c->dwork);
+    HIPCHK(c, hipGetLastError());
+    return PT_OK;
+}
+
+int pt_compact(pt_ctx* c, int sp) {
+    if (!c) return PT_E_ARG;
+    if (sp < -1 || sp >= PT_NSPECIES) return fail(c, PT_E_ARG, "bad species");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = read_counts(c);
+    if (rc) return rc;
+    if (c->profile) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    for (int s = 0; s < PT_NSPECIES; s++) {
+        if (sp >= 0 && s != sp) continue;
+        rc = compact_species(c, s);
+        if (rc) return rc;
+    }
+    if (c->profile) {
+        HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev[4]));
+        float a = 0;
+        HIPCHK(c, hipEventElapsedTime(&a, c->ev[3], c->ev[4]));
+        c->cmp_ms += a;
+    }
+    return read_counts(c);
+}
+
+int pt_run(pt_ctx* c, double t0, double dt, int nsteps, int compact_every, int check_every) {
+    if (!c) return PT_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = require_tables(c);
+    if (rc) return rc;
+    rc = sync_params(c);
+    if (rc) return rc;
+    if (check_every < 1) check_every = 1;
+    double t = t0;
+    for (int q = 0; q < nsteps; q++) {
+        rc = launch_step(c, t + dt);
+        if (rc) return rc;
+        t += dt;
+        bool last = q == nsteps - 1;
+        if (compact_every > 0 && (q + 1) % compact_every == 0) {
+            rc = pt_compact(c, -1);
+            if (rc) return rc;
+        }
+        if (last || (q + 1) % check_every == 0) {
+            rc = read_counts(c);
+            if (rc) return rc;
+            rc = check_flags(c, nullptr);
+            if (rc) return rc;
+        }
+    }
+    return PT_OK;
+}
+
+int pt_stats(pt_ctx* c, int sp, pt_pop_stats* out) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES || !out) return fail(c, PT_E_ARG, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = sync_params(c);
+    if (rc) return rc;
+    rc = read_counts(c);
+    if (rc) return rc;
+    int blocks = (int)std::min<ull>(4096, std::max<ull>(1, (c->hn[sp] + 255) / 256));
+    hipLaunchKernelGGL(k_stats_partial, dim3(blocks), dim3(256), 0, c->stream, c->dp, sp, c->stat_part);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> part((size_t)blocks * NSTAT);
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->stat_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double a[NSTAT] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < blocks; b++)
+        for (int q = 0; q < NSTAT; q++) {
+            double v = part[(size_t)b * NSTAT + q];
+            if (q == 7) a[q] = std::max(a[q], v);
+            else a[q] += v;
+        }
+    out->n = c->hn[sp];
+    out->nactive = (uint64_t)a[0];
+    out->weight = a[1];
+    out->mean_energy = a[1] > 0 ? a[2] / a[1] : 0.0;
+    for (int d = 0; d < 3; d++) out->centroid[d] = a[1] > 0 ? a[3 + d] / a[1] : 0.0;
+    double xm2 = (out->centroid[0] * out->centroid[0] + out->centroid[1] * out->centroid[1]) +
+                 out->centroid[2] * out->centroid[2];
+    out->spread = a[1] > 0 ? std::sqrt(std::fabs(a[6] / a[1] - xm2)) : 0.0;
+    out->max_energy = a[7];
+    return PT_OK;
+}
+
+int pt_collision_counts(pt_ctx* c, int sp, int64_t* out, int nmax, int reset) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES || !out || nmax < 1) return fail(c, PT_E_ARG, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    ull h[NCOUNT];
+    HIPCHK(c, hipMemcpyAsync(h, c->dcounts + sp * NCOUNT, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int q = 0; q < std::min(nmax, NCOUNT); q++) out[q] = (int64_t)h[q];
+    if (reset) HIPCHK(c, hipMemsetAsync(c->dcounts + sp * NCOUNT, 0, sizeof(h), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+int pt_tally_read(pt_ctx* c, int sp, double* hist, int nbins, double* count, double* weight) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES) return fail(c, PT_E_ARG, "bad species");
+    HIPCHK(c, hipSetDevice(c->device));
+    int b = c->hp.tally_bins;
+    std::vector<double> h(b + 2);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->dtally + (size_t)sp * (b + 2), sizeof(double) * (b + 2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (hist)
+        for (int q = 0; q < std::min(b, nbins); q++) hist[q] = h[q];
+    if (count) *count = h[b];
+    if (weight) *weight = h[b + 1];
+    return PT_OK;
+}
+
+int pt_work_read(pt_ctx* c, pt_work* out, int reset) {
+    if (!c) return PT_E_ARG;
+    if (!out) return fail(c, PT_E_ARG, "null output");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = drain_events(c);
+    if (rc) return rc;
+    ull w[NWORK];
+    HIPCHK(c, hipMemcpyAsync(w, c->dwork, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    memset(out, 0, sizeof(*out));
+    out->particle_steps = w[W_PSTEPS];
+    out->secondaries = w[W_SECONDARIES];
+    out->moved = w[W_MOVED];
+    out->events = w[W_EVENTS];
+    out->tail_particles = w[W_TAIL];
+    out->steps = c->step;
+    out->advance_ms = c->adv_ms;
+    out->advance_launches = c->adv_launches;
+    out->tail_ms = c->tail_ms;
+    out->compact_ms = c->cmp_ms;
+    out->main_secondaries = w[W_MAIN_SEC];
+    if (reset) {
+        HIPCHK(c, hipMemsetAsync(c->dwork, 0, sizeof(w), c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->adv_ms = c->tail_ms = c->cmp_ms = 0.0;
+        c->adv_launches = 0;
+    }
+    return PT_OK;
+}
+
+int pt_profile_enable(pt_ctx* c, int on) {
+    if (!c) return PT_E_ARG;
+    c->profile = on != 0;
+    return PT_OK;
+}
+
+int pt_set_step(pt_ctx* c, uint32_t step) {
+    if (!c) return PT_E_ARG;
+    c->step = step;
+    return PT_OK;
+}
+
+int pt_get_step(pt_ctx* c, uint32_t* step) {
+    if (!c || !step) return PT_E_ARG;
+    *step = c->step;
+    return PT_OK;
+}
+
+int pt_synchronize(pt_ctx* c) {
+    if (!c) return PT_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+// ---- test hooks
+
+#define SCRATCH(c, var, T, count)                                                             \
+    Scratch<T> var;                                                                           \
+    do {                                                                                      \
+        hipError_t e_ = hipMalloc((void**)&var.p, sizeof(T) * ((count) ? (count) : 1));      \
+        if (e_ != hipSuccess) return fail((c), PT_E_NOMEM, "scratch allocation failed");      \
+    } while (0)
+
+int pt_debug_philox(pt_ctx* c, const uint32_t* ctr, const uint32_t key[2], uint32_t* out, size_t n) {
+    if (!c) return PT_E_ARG;
+    if (!ctr || !key || !out) return fail(c, PT_E_ARG, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    SCRATCH(c, dc, uint32_t, 4 * n);
+    SCRATCH(c, dout, uint32_t, 4 * n);
+    HIPCHK(c, hipMemcpy(dc.p, ctr, sizeof(uint32_t) * 4 * n, hipMemcpyHostToDevice));
+    if (n) {
+        hipLaunchKernelGGL(k_dbg_philox, dim3((n + 255) / 256), dim3(256), 0, c->stream, dc.p, key[0], key[1], dout.p, (ull)n);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, dout.p, sizeof(uint32_t) * 4 * n, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+int pt_debug_uniforms(pt_ctx* c, uint64_t uid, uint32_t step, double* out, size_t n) {
+    if (!c) return PT_E_ARG;
+    if (!out) return fail(c, PT_E_ARG, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    SCRATCH(c, d, double, n);
+    hipLaunchKernelGGL(k_dbg_uniforms, dim3(1), dim3(64), 0, c->stream, c->cfg.seed, uid, step, d.p, (ull)n);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, d.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+int pt_debug_select(pt_ctx* c, int sp, size_t n, const double* E, const double* u, const int32_t* swarm,
+                    int32_t* interval, int32_t* proc, double* rbound) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES || !E || !u || !interval || !proc || !rbound)
+        return fail(c, PT_E_ARG, "bad arguments");
+    if (!c->hp.tab[sp].valid) return fail(c, PT_E_STATE, "no table for species %d", sp);
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = sync_params(c);
+    if (rc) return rc;
+    SCRATCH(c, dE, double, n);
+    SCRATCH(c, du, double, n);
+    SCRATCH(c, dsw, int32_t, n);
+    SCRATCH(c, di, int32_t, n);
+    SCRATCH(c, dp, int32_t, n);
+    SCRATCH(c, dr, double, n);
+    HIPCHK(c, hipMemcpy(dE.p, E, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(du.p, u, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (swarm) HIPCHK(c, hipMemcpy(dsw.p, swarm, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    if (n) {
+        hipLaunchKernelGGL(k_dbg_select, dim3((n + 255) / 256), dim3(256), c->lds_bytes, c->stream, c->dp, sp,
+                           (ull)n, dE.p, du.p, swarm ? dsw.p : nullptr, di.p, dp.p, dr.p);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(interval, di.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(proc, dp.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(rbound, dr.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+int pt_debug_push(pt_ctx* c, int sp, size_t n, double* const soa[PT_NFIELDS], const double* dt) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES || !soa || !dt) return fail(c, PT_E_ARG, "bad arguments");
+    const int fi[7] = {PT_X, PT_Y, PT_Z, PT_PX, PT_PY, PT_PZ, PT_T};
+    for (int q = 0; q < 7; q++)
+        if (!soa[fi[q]]) return fail(c, PT_E_ARG, "push needs x, p and t");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = sync_params(c);
+    if (rc) return rc;
+    SCRATCH(c, buf, double, 8 * n);
+    for (int q = 0; q < 7; q++)
+        HIPCHK(c, hipMemcpy(buf.p + (size_t)q * n, soa[fi[q]], sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(buf.p + 7 * n, dt, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (n) {
+        double* b = buf.p;
+        hipLaunchKernelGGL(k_dbg_push, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->dp, sp, (ull)n, b,
+                           b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, b + 6 * n, b + 7 * n);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 7; q++)
+        HIPCHK(c, hipMemcpy(soa[fi[q]], buf.p + (size_t)q * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+int pt_debug_collide(pt_ctx* c, int sp, int proc, size_t n, const double* const soa[PT_NFIELDS],
+                     const uint64_t* uid, const double* u, int stride, int32_t* out_kind, int32_t* out_draws,
+                     double* out_states, int32_t* out_species, uint64_t* out_uid) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES || !soa || !uid || !u || stride < 1 || !out_kind || !out_draws ||
+        !out_states || !out_species || !out_uid)
+        return fail(c, PT_E_ARG, "bad arguments");
+    if (!c->hp.tab[sp].valid || proc < 0 || proc >= c->hp.tab[sp].nprocs)
+        return fail(c, PT_E_ARG, "process %d not in the table of species %d", proc, sp);
+    for (int f = 0; f < 8; f++)
+        if (!soa[f]) return fail(c, PT_E_ARG, "collide needs x, p, w, t");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = require_tables(c);
+    if (rc) return rc;
+    rc = sync_params(c);
+    if (rc) return rc;
+    std::vector<double> in(n * PT_NFIELDS, 0.0);
+    for (size_t i = 0; i < n; i++)
+        for (int f = 0; f < 8; f++) in[i * PT_NFIELDS + f] = soa[f][i];
+    SCRATCH(c, din, double, n * PT_NFIELDS);
+    SCRATCH(c, duid, uint64_t, n);
+    SCRATCH(c, du, double, n * (size_t)stride);
+    SCRATCH(c, dk, int32_t, n);
+    SCRATCH(c, dd, int32_t, n);
+    SCRATCH(c, dst, double, n * 3 * PT_NFIELDS);
+    SCRATCH(c, dsp, int32_t, n * 3);
+    SCRATCH(c, dou, uint64_t, n * 3);
+    HIPCHK(c, hipMemcpy(din.p, in.data(), sizeof(double) * n * PT_NFIELDS, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(duid.p, uid, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(du.p, u, sizeof(double) * n * stride, hipMemcpyHostToDevice));
+    if (n) {
+        hipLaunchKernelGGL(k_dbg_collide, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->dp, sp, proc, (ull)n,
+                           din.p, duid.p, du.p, stride, c->step, dk.p, dd.p, dst.p, dsp.p, dou.p);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out_kind, dk.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(out_draws, dd.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(out_states, dst.p, sizeof(double) * n * 3 * PT_NFIELDS, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(out_species, dsp.p, sizeof(int32_t) * n * 3, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(out_uid, dou.p, sizeof(uint64_t) * n * 3, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,762 @@
+// pt_device.h -- device code of the null-collision step: constants, the uid-keyed Philox
+// stream, kinematics, the binary-interval Chebyshev rate tables, the RK2 push and every
+// collision sampler.  Compiled for gfx950 with -ffp-contract=off: the arithmetic follows
+// the reference's Julia evaluation order (a*b*c == (a*b)*c, x^2 == x*x, x^3 == x*x*x,
+// left-fold sums), so selection, indices and pushes are bit-identical to a CPU
+// restatement of the same code; transcendental functions come from the ROCm device
+// library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/particulator_hip.h"
+
+namespace pt {
+
+// ---------------------------------------------------------------- constants
+// src/constants.jl:6-166 and the derived constants :157-164
+constexpr double PI = 3.141592653589793;
+constexpr double C = 299792458.0;
+constexpr double C2 = C * C;
+constexpr double ME = 9.10938356e-31;
+constexpr double QE = 1.6021766208e-19;
+constexpr double ALPHA = 0.0072973525664;
+constexpr double HBAR = 1.0545718001391127e-34;
+constexpr double EPS0 = 8.854187817620389e-12;
+constexpr double EV = 1.6021766208e-19;
+constexpr double MC2 = ME * C2;
+constexpr double RE = QE * QE / (ME * C2) / (4.0 * PI * EPS0);
+constexpr double A0 = HBAR / (ME * C * ALPHA);
+constexpr double TWO_PI = 2.0 * PI;
+constexpr double EPS64 = 2.220446049250313e-16;  // eps(Float64), src/mixed_population.jl:65
+
+constexpr int MAX_PROCS = 24;
+constexpr int MAX_INTERVALS = 64;
+constexpr int MAX_SHELLS = 8;
+constexpr int MAX_SB = 4;
+constexpr int MAX_PE = 4;
+// Hard limits that bound every loop of a particle's step: uniforms drawn and loop
+// iterations.  Reaching one sets the sampler error flag and ends the particle's step.
+constexpr uint32_t DRAW_LIMIT = 1u << 24;
+constexpr uint32_t ITER_LIMIT = 1u << 22;
+
+// ---------------------------------------------------------------- device tables
+struct DevProc {
+    int kind, Z, slot, N;
+    double B, U, tcut, zf;
+    double c[4];
+};
+
+struct DevTable {
+    int nprocs, k, nswarm, valid;
+    double xmax, cut;
+    const double* rate;       // [(k+1)][L][3]
+    const double* ratebound;  // [nswarm][(k+1)][3]
+    DevProc procs[MAX_PROCS];
+};
+
+struct DevSB {
+    int ne, ncum;
+    const double* log_energy;  // [ne]
+    const double* data;        // [ne][ncum]
+};
+
+struct DevPE {
+    int nshell;
+    double binding[MAX_SHELLS];
+};
+
+struct FieldFn {
+    int kind, pad;
+    double v[3], v2[3];
+    double z1, z2, sx, sy, sz, ez0;
+};
+struct Field {
+    FieldFn e, b;
+};
+
+// ---------------------------------------------------------------- Philox4x32-10
+// Salmon et al., SC'11: multipliers 0xD2511F53, 0xCD9E8D57; key increments 0x9E3779B9,
+// 0xBB67AE85; ten rounds.
+__device__ __forceinline__ void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3,
+                                         uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+}
+
+// 64 random bits -> (0,1): top 52 bits + 1/2 (exact)
+__device__ __forceinline__ double u01(uint32_t lo, uint32_t hi) {
+    uint64_t b = ((uint64_t)hi << 32) | lo;
+    return ((double)(b >> 12) + 0.5) * 0x1p-52;
+}
+
+// uniform stream of one particle in one step: counter (uid, step, draw/2), key = seed
+struct PhiloxStream {
+    uint32_t k0, k1, u0, u1, step;
+    uint32_t draw;
+    uint32_t cblk;
+    double c0, c1;
+    __device__ bool exhausted() const { return draw >= DRAW_LIMIT; }
+    __device__ void init(uint64_t seed, uint64_t uid, uint32_t st) {
+        k0 = (uint32_t)seed;
+        k1 = (uint32_t)(seed >> 32);
+        u0 = (uint32_t)uid;
+        u1 = (uint32_t)(uid >> 32);
+        step = st;
+        draw = 0;
+        cblk = 0;
+    }
+    __device__ __forceinline__ double next() {
+        uint32_t d = draw++;
+        uint32_t blk = d >> 1;
+        if (cblk != blk + 1) {
+            uint32_t a = u0, b = u1, c = step, e = blk;
+            philox10(a, b, c, e, k0, k1);
+            c0 = u01(a, b);
+            c1 = u01(c, e);
+            cblk = blk + 1;
+        }
+        return (d & 1) ? c1 : c0;
+    }
+};
+
+// injected uniforms (test hook): consumed in order, 0.5 once exhausted
+struct InjectedStream {
+    const double* u;
+    int n, pos;
+    uint32_t draw;
+    bool ran_out;
+    __device__ bool exhausted() const { return ran_out; }
+    __device__ __forceinline__ double next() {
+        draw++;
+        if (pos >= n) {
+            ran_out = true;
+            return 0.5;
+        }
+        return u[pos++];
+    }
+};
+
+// uid of the k-th particle created by the event that ended at draw `draw` of the stream
+// (uid, step): a Philox counter with the top bit of word 3 set (never a uniform draw)
+__device__ __forceinline__ uint64_t child_uid(uint64_t seed, uint64_t uid, uint32_t step,
+                                              uint32_t draw, int k) {
+    uint32_t a = (uint32_t)uid, b = (uint32_t)(uid >> 32), c = step;
+    uint32_t e = 0x80000000u | ((draw & 0x3FFFFFFFu) << 1) | (uint32_t)(k & 1);
+    philox10(a, b, c, e, (uint32_t)seed, (uint32_t)(seed >> 32));
+    return ((uint64_t)b << 32) | a;
+}
+
+template <class G>
+__device__ __forceinline__ double nextcoll(G& g) {  // src/util.jl:17
+    return -log(g.next());
+}
+
+// ---------------------------------------------------------------- kinematics
+struct V3 {
+    double x, y, z;
+};
+__device__ __forceinline__ double dot3(const V3& a, const V3& b) {
+    return (a.x * b.x + a.y * b.y) + a.z * b.z;
+}
+__device__ __forceinline__ double norm3(const V3& a) { return sqrt(dot3(a, a)); }
+
+// src/electron.jl:53, src/photon.jl:52
+__device__ __forceinline__ double kinenergy(int sp, const V3& p) {
+    double pp = dot3(p, p);
+    if (sp == PT_PHOTON) return sqrt(pp) * C;
+    return sqrt(MC2 * MC2 + C2 * pp) - MC2;
+}
+// src/electron.jl:51
+__device__ __forceinline__ double mom_from_kin(int sp, double kin) {
+    if (sp == PT_PHOTON) return kin / C;
+    double a = kin + MC2;
+    return sqrt(a * a - MC2 * MC2) / C;
+}
+// src/electron.jl:54-56, src/photon.jl:47
+__device__ __forceinline__ V3 velocity(int sp, const V3& p) {
+    if (sp == PT_PHOTON) {
+        double f = C / norm3(p);
+        return V3{p.x * f, p.y * f, p.z * f};
+    }
+    double g = sqrt(1.0 + C2 * dot3(p, p) / (MC2 * MC2));
+    double inv = 1.0 / (ME * g);
+    return V3{p.x * inv, p.y * inv, p.z * inv};
+}
+
+// ---------------------------------------------------------------- Chebyshev
+// precheb, src/cheby.jl:127-143 (+ chebval :57-81).  The index is clamped to [0, k]
+// (the reference reads out of bounds above xmax); `oob` reports it.
+__device__ __forceinline__ int precheb(double x, int k, double xmax, double& t1, double& t2,
+                                       bool& oob) {
+    double x1 = x / xmax;
+    int l = 0;
+    double s;
+    if (x1 != x1) {
+        s = x1;
+        l = 0;
+    } else {
+        s = frexp(x1, &l);
+    }
+    int i = (x1 == 0.0) ? 0 : l + k;
+    double xi;
+    if (i > 0) {
+        xi = 4.0 * s - 3.0;
+    } else {
+        xi = ldexp(1.0, k + 1) * x1 - 1.0;
+        i = 0;
+    }
+    t1 = xi;
+    t2 = 2.0 * xi * xi - 1.0;
+    oob = i > k;
+    return i > k ? k : i;
+}
+
+// sum(ntuple(k -> c[k]*t[k], Val(3))), src/collision_table.jl:318,332
+__device__ __forceinline__ double cheb3(const double* c, double t1, double t2) {
+    return (c[0] * 1.0 + c[1] * t1) + c[2] * t2;
+}
+
+// ---------------------------------------------------------------- turn, sphere
+// turn, src/util.jl:40-57.  Where the reference yields NaN, 1 - cos^2 < 0 (rounding) is
+// clamped to 0 and an exactly axial direction (s == 0) uses the axis-aligned rotation.
+__device__ __forceinline__ V3 turn(const V3& u, double cost, double phi, double n) {
+    double nrm = norm3(u);
+    double m0 = u.x / nrm, m1 = u.y / nrm, m2 = u.z / nrm;
+    double sp, cp;
+    sincos(phi, &sp, &cp);
+    double q = 1.0 - cost * cost;
+    double sint = sqrt(q > 0.0 ? q : 0.0);
+    double qs = 1.0 - m2 * m2;
+    double s = sqrt(qs > 0.0 ? qs : 0.0);
+    double x, y, z;
+    if (s > 0.0) {
+        double bx = m0 * m2 * cp + (-m1) * sp;
+        double by = m1 * m2 * cp + m0 * sp;
+        x = sint * bx / s + m0 * cost;
+        y = sint * by / s + m1 * cost;
+        z = -s * sint * cp + m2 * cost;
+    } else {
+        x = sint * cp;
+        y = sint * sp;
+        z = m2 * cost;
+    }
+    return V3{n * x, n * y, n * z};
+}
+
+// randsphere, src/util.jl:4-12
+template <class G>
+__device__ __forceinline__ V3 randsphere(G& g) {
+    double phi = TWO_PI * g.next();
+    double sp, cp;
+    sincos(phi, &sp, &cp);
+    double u = 2.0 * g.next() - 1.0;
+    double v = sqrt(1.0 - u * u);
+    return V3{v * cp, v * sp, u};
+}
+
+// ---------------------------------------------------------------- samplers
+// sample_modified_tsai_cos_theta, src/util.jl:143-159
+template <class G>
+__device__ double sample_tsai(double T, G& g) {
+    double umax = 2.0 * (1.0 + T / (ME * C2));
+    const double a1 = 1.6, a2 = 1.6 / 3.0, border = 0.25;
+    double u;
+    for (;;) {
+        double r1 = g.next();
+        double r2 = g.next();
+        double uu = -log(r1 * r2);
+        u = border > g.next() ? uu * a1 : uu * a2;
+        if (u <= umax || g.exhausted()) break;
+    }
+    return 1.0 - 2.0 * (u * u) / (umax * umax);
+}
+
+// sample_rel_sr, src/relativistic_coulomb.jl:35-51
+template <class G>
+__device__ double sample_rel_sr(double alpha, double beta, G& g) {
+    double x;
+    for (;;) {
+        double u = g.next();
+        x = alpha * u / (alpha + 1.0 - u);
+        double z = g.next();
+        if (z < 1.0 - beta * beta * x || g.exhausted()) break;
+    }
+    return 1.0 - 2.0 * x;
+}
+
+// rbeb_sample / _rbeb_sample, src/rbeb.jl:156-196
+template <class G>
+__device__ double rbeb_sample(double T, double B, G& g) {
+    double t1 = T / (ME * C2);
+    double b1 = B / (ME * C2);
+    double bt2 = 1.0 - 1.0 / ((1.0 + t1) * (1.0 + t1));
+    double t = T / B;
+    double A = -(1.0 + 2.0 * t1) / (t + 1.0) / ((1.0 + t1) * (1.0 + t1));
+    const double Bc = 1.0;
+    double Cc = (log(bt2 / (1.0 - bt2)) - bt2) - log(2.0 * b1);
+    double M = b1 * b1 / ((1.0 + t1) * (1.0 + t1));
+    double pbn = 2.0 * Bc + 2.0 * Cc + (t + 1.0) * (t + 1.0) * Bc * M / 4.0;
+    double ratio = (t + 1.0) / (t - 1.0);
+    double w;
+    for (;;) {
+        double u = g.next();
+        w = u / (ratio - u);
+        double w1 = 1.0 + w, tw = t - w;
+        double pb = pbn / (w1 * w1);
+        double wa = w + 1.0;
+        double g1 = 1.0 / wa + 1.0 / tw;
+        double g2 = 1.0 / (wa * wa) + 1.0 / (tw * tw);
+        double g3 = 1.0 / (wa * wa * wa) + 1.0 / (tw * tw * tw);
+        double p0 = A * g1 + Bc * (g2 + M) + Cc * g3;
+        if (g.next() * pb < p0 || g.exhausted()) break;
+    }
+    return B * w;
+}
+
+// sample_secondary_energy(::Moller), src/moller.jl:64-87
+template <class G>
+__device__ double moller_sample(double tcut, double eng, G& g) {
+    double e0 = tcut / eng;
+    double gm = 1.0 + eng / MC2;
+    double pre = 4.0 / (9.0 * (gm * gm) - 10.0 * gm + 5.0);
+    double ca = (gm - 1.0) * (gm - 1.0);
+    double cb = 2.0 * (gm * gm) + 2.0 * gm - 1.0;
+    double cc = gm * gm;
+    double eps;
+    for (;;) {
+        double r = g.next();
+        eps = e0 / (1.0 - r + 2.0 * e0 * r);
+        double om = 1.0 - eps;
+        double gg = pre * (ca * (eps * eps) - cb * (eps / om) + cc / (om * om));
+        if (g.next() < gg || g.exhausted()) break;
+    }
+    return eps * eng;
+}
+
+// bhaba_bs, src/bhaba.jl:84-91
+__device__ __forceinline__ void bhabha_bs(double y, double& B1, double& B2, double& B3,
+                                          double& B4) {
+    double q = 1.0 - 2.0 * y;
+    B1 = 2.0 - y * y;
+    B2 = q * (3.0 + y * y);
+    B3 = q * q + q * q * q;
+    B4 = q * q * q;
+}
+
+// sample_secondary_energy(::Bhaba), src/bhaba.jl:55-81
+template <class G>
+__device__ double bhabha_sample(double tcut, double eng, G& g) {
+    double e0 = tcut / eng;
+    double gm = 1.0 + eng / MC2;
+    double y = 1.0 / (gm + 1.0);
+    double B0 = gm * gm / (gm * gm - 1.0);
+    double B1, B2, B3, B4;
+    bhabha_bs(y, B1, B2, B3, B4);
+    double g2 = (((B0 + B1 * e0) + B2 * (e0 * e0)) + B3 * (e0 * e0 * e0)) + B4 * pow(e0, 4.0);
+    double eps;
+    for (;;) {
+        double r = g.next();
+        eps = e0 / (1.0 - r + e0 * r);
+        double g1 = (((B0 + B1 * eps) + B2 * (eps * eps)) + B3 * (eps * eps * eps)) +
+                    B4 * pow(eps, 4.0);
+        double gg = g1 / g2;
+        if (g.next() < gg || g.exhausted()) break;
+    }
+    return eps * eng;
+}
+
+// searchsortedfirst(LinRange(0,1,ncum), x): first i (1-based) with (i-1)/(ncum-1) >= x
+__device__ __forceinline__ int pcum_first(double x, int ncum) {
+    double d = (double)(ncum - 1);
+    int i = (int)(x * d) + 1;
+    if (i < 1) i = 1;
+    if (i > ncum) i = ncum;
+    while (i > 1 && (double)(i - 2) / d >= x) i--;
+    while (i < ncum && (double)(i - 1) / d < x) i++;
+    return i;
+}
+
+// sample_secondary_energy(::SeltzerBerger), src/seltzer.jl:97-122.  Returns k/T, or a
+// negative value where the reference indexes out of bounds.
+template <class G>
+__device__ double sb_sample(const DevSB& sb, double T, G& g) {
+    double x = g.next();
+    double y = log(T);
+    int ncum = sb.ncum;
+    double d = (double)(ncum - 1);
+    int i2 = pcum_first(x, ncum), i1 = i2 - 1;
+    // searchsortedfirst(log_energy, y) by bisection (1-based result)
+    int lo = 0, hi = sb.ne;  // first index in [0, ne] with log_energy[idx] >= y
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (sb.log_energy[mid] < y) lo = mid + 1;
+        else hi = mid;
+    }
+    int j2 = lo + 1, j1 = j2 - 1;
+    if (i1 < 1 || j1 < 1 || j2 > sb.ne) return -1.0;
+    double x1 = (double)(i1 - 1) / d, x2 = (double)(i2 - 1) / d;
+    double y1 = sb.log_energy[j1 - 1], y2 = sb.log_energy[j2 - 1];
+    const double* c1 = sb.data + (size_t)(j1 - 1) * ncum;
+    const double* c2 = sb.data + (size_t)(j2 - 1) * ncum;
+    double A = (x2 - x1) * (y2 - y1);
+    double S = c1[i1 - 1] * (x2 - x) * (y2 - y) + c2[i1 - 1] * (x - x1) * (y2 - y);
+    S = S + c1[i2 - 1] * (x2 - x) * (y - y1);
+    S = S + c2[i2 - 1] * (x - x1) * (y - y1);
+    return exp(S / A);
+}
+
+// sample_secondary_energy_and_cos_theta(::Compton), src/compton.jl:119-144
+template <class G>
+__device__ void compton_sample(double eng, G& g, double& E1, double& cost) {
+    double e0 = MC2 / (MC2 + 2.0 * eng);
+    double a1 = -log(e0);
+    double a2 = (1.0 - e0 * e0) / 2.0;
+    double thr = a1 / (a1 + a2);
+    double t, eps;
+    for (;;) {
+        if (g.next() < thr) eps = exp(-g.next() * a1);
+        else eps = sqrt(e0 * e0 + (1.0 - e0 * e0) * g.next());
+        t = MC2 * (1.0 - eps) / (eps * eng);
+        double gg = 1.0 - eps / (1.0 + eps * eps) * t * (2.0 - t);
+        if (g.next() < gg || g.exhausted()) break;
+    }
+    cost = 1.0 - t;
+    E1 = eps * eng;
+}
+
+// sample_electron_cos_theta(::PhotoElectric), src/photo_electric.jl:78-99
+template <class G>
+__device__ double sauter_sample(double ee, G& g) {
+    double gm = 1.0 + ee / MC2;
+    double b = sqrt(1.0 - 1.0 / (gm * gm));
+    double A = 1.0 / b - 1.0;
+    double k2 = b * gm * (gm - 1.0) * (gm - 2.0) / 2.0;
+    double g0 = 2.0 * (1.0 / A + k2);
+    double A2 = (A + 2.0) * (A + 2.0);
+    double nu;
+    for (;;) {
+        double xi = g.next();
+        nu = 2.0 * A / (A2 - 4.0 * xi) * (2.0 * xi + (A + 2.0) * sqrt(xi));
+        double xi1 = g.next();
+        double gn = (2.0 - nu) * (1.0 / (A + nu) + k2);
+        if (xi1 * g0 < gn || g.exhausted()) break;
+    }
+    return 1.0 - nu;
+}
+
+// _screen_function1/2, src/bethe_heitler.jl:263-293
+__device__ __forceinline__ double screen1(double d) {
+    return d > 1.4 ? 42.038 - 8.29 * log(d + 0.958) : 42.184 - d * (7.444 - 1.623 * d);
+}
+__device__ __forceinline__ double screen2(double d) {
+    return d > 1.4 ? 42.038 - 8.29 * log(d + 0.958) : 41.326 - d * (5.848 - 0.902 * d);
+}
+__device__ __forceinline__ double jmax(double a, double b) {
+    if (a != a || b != b) return a + b;  // NaN propagates as in Julia's max
+    return a > b ? a : b;
+}
+
+// sample_secondary_energy(::BetheHeitler), src/bethe_heitler.jl:185-246.  cz = Z^(1/3),
+// fz_lo = 8 log(Z)/3, fz_hi = fz_lo + 8 fc (precomputed by the host).  Returns the
+// reference's (ekin, pkin).
+template <class G>
+__device__ void bh_sample(double cz, double fz_lo, double fz_hi, double eng, G& g,
+                          double& ekin, double& pkin) {
+    double e0 = MC2 / eng;
+    double eps;
+    if (eng < 2e6 * EV) {
+        eps = e0 + (0.5 - e0) * g.next();
+    } else {
+        double d0 = 136.0 * e0 / cz;
+        double FZ = eng > 50e6 * EV ? fz_hi : fz_lo;
+        double dmin = 4.0 * d0;
+        double dmax = exp((42.24 - FZ) / 8.368) - 0.952;
+        double ep = (1.0 - sqrt(1.0 - dmin / dmax)) / 2.0;
+        double emin = e0 > ep ? e0 : ep;
+        double erange = 0.5 - emin;
+        double F10 = screen1(dmin) - FZ, F20 = screen2(dmin) - FZ;
+        double NF1 = jmax(F10 * (erange * erange), 0.0);
+        double NF2 = jmax(1.5 * F20, 0.0);
+        double NC = NF1 / (NF1 + NF2);
+        for (;;) {
+            bool acc;
+            if (NC > g.next()) {
+                eps = 0.5 - erange * pow(g.next(), 1.0 / 3.0);
+                double d = d0 / (eps * (1.0 - eps));
+                acc = g.next() < (screen1(d) - FZ) / F10;
+            } else {
+                eps = emin + erange * g.next();
+                double d = d0 / (eps * (1.0 - eps));
+                acc = g.next() < (screen2(d) - FZ) / F20;
+            }
+            if (acc || g.exhausted()) break;
+        }
+    }
+    double etot, ptot;
+    if (g.next() < 0.5) {  // rand(Bool), src/bethe_heitler.jl:233
+        etot = (1.0 - eps) * eng;
+        ptot = eps * eng;
+    } else {
+        ptot = (1.0 - eps) * eng;
+        etot = eps * eng;
+    }
+    ekin = jmax(0.0, etot - MC2);
+    pkin = jmax(0.0, ptot - MC2);
+}
+
+// sample_secondary_energy_fraction(::PositronAnihilation), src/anihilation.jl:39-59
+template <class G>
+__device__ double annih_sample(double eng, G& g) {
+    double gm = 1.0 + eng / MC2;
+    double sq = sqrt((gm - 1.0) / (gm + 1.0));
+    double emax = (1.0 + sq) / 2.0, emin = (1.0 - sq) / 2.0;
+    double ratio = emax / emin;
+    double den = (gm + 1.0) * (gm + 1.0);
+    double eps;
+    for (;;) {
+        eps = emin * pow(ratio, g.next());
+        double gg = 1.0 - eps + (2.0 * gm * eps - 1.0) / (eps * den);
+        if (g.next() < gg || g.exhausted()) break;
+    }
+    return eps;
+}
+
+// ---------------------------------------------------------------- fields and push
+__device__ __forceinline__ V3 field_eval(const FieldFn& f, const V3& x) {
+    switch (f.kind) {
+    case PT_FIELD_HOMOGENEOUS:
+        return V3{f.v[0], f.v[1], f.v[2]};
+    case PT_FIELD_DOUBLELAYER:
+        if (f.z1 < x.z && x.z < f.z2) return V3{f.v[0], f.v[1], f.v[2]};
+        return V3{0.0, 0.0, 0.0};
+    case PT_FIELD_STEP:
+        if (x.z < f.z1) return V3{f.v[0], f.v[1], f.v[2]};
+        return V3{f.v2[0], f.v2[1], f.v2[2]};
+    case PT_FIELD_CONFINED: {
+        double g = exp(-(x.x * x.x / (2.0 * (f.sx * f.sx)) + x.y * x.y / (2.0 * (f.sy * f.sy)) +
+                         x.z * x.z / (2.0 * (f.sz * f.sz))));
+        return V3{-(f.ez0 * g * x.x * x.z) / (f.sx * f.sx), -(f.ez0 * g * x.y * x.z) / (f.sy * f.sy),
+                  f.ez0 * g - (f.ez0 * g * (x.z * x.z)) / (f.sz * f.sz)};
+    }
+    default:
+        return V3{0.0, 0.0, 0.0};
+    }
+}
+
+// force(::ElectromagneticField, s), src/field.jl:62-68
+__device__ __forceinline__ V3 force(int sp, const Field& f, const V3& x, const V3& p) {
+    if (sp == PT_PHOTON) return V3{0.0, 0.0, 0.0};
+    V3 e = field_eval(f.e, x), b = field_eval(f.b, x);
+    V3 v = velocity(sp, p);
+    double q = (sp == PT_ELECTRON ? -1.0 : 1.0) * QE;
+    double cx = v.y * b.z - v.z * b.y;
+    double cy = v.z * b.x - v.x * b.z;
+    double cz = v.x * b.y - v.y * b.x;
+    return V3{q * (e.x + cx), q * (e.y + cy), q * (e.z + cz)};
+}
+
+// advance_particle(::RK2Pusher, y, dt), src/pusher.jl:41-63
+__device__ __forceinline__ void push(int sp, const Field& f, V3& x, V3& p, double& t, double dt) {
+    V3 v1 = velocity(sp, p);
+    V3 f1 = force(sp, f, x, p);
+    double dt2 = 2.0 * dt;
+    V3 x2{x.x + dt2 * v1.x / 3.0, x.y + dt2 * v1.y / 3.0, x.z + dt2 * v1.z / 3.0};
+    V3 p2{p.x + dt2 * f1.x / 3.0, p.y + dt2 * f1.y / 3.0, p.z + dt2 * f1.z / 3.0};
+    V3 v2 = velocity(sp, p2);
+    V3 f2 = force(sp, f, x2, p2);
+    x.x = x.x + dt * (v1.x / 4.0 + 3.0 * v2.x / 4.0);
+    x.y = x.y + dt * (v1.y / 4.0 + 3.0 * v2.y / 4.0);
+    x.z = x.z + dt * (v1.z / 4.0 + 3.0 * v2.z / 4.0);
+    p.x = p.x + dt * (f1.x / 4.0 + 3.0 * f2.x / 4.0);
+    p.y = p.y + dt * (f1.y / 4.0 + 3.0 * f2.y / 4.0);
+    p.z = p.z + dt * (f1.z / 4.0 + 3.0 * f2.z / 4.0);
+    t = t + dt;
+}
+
+// ---------------------------------------------------------------- collide
+// The outcome of one real collision: the primary's new state (if it survives) and up to
+// two new particles (src/collisions.jl:11-55).
+enum { OUT_NULL = 0, OUT_STATECHANGE = 1, OUT_NEW = 2, OUT_REPLACE = 3, OUT_REPLACE_PAIR = 4 };
+
+struct NewPart {
+    V3 p;
+    double s;
+    int sp;
+    uint64_t uid;
+};
+
+struct Outcome {
+    int kind;
+    int nnew;
+    V3 p0;      // primary's new momentum (STATECHANGE, NEW)
+    double s0;  // primary's new s
+    NewPart q[2];
+    int err;
+};
+
+// Lehtinen 1999 angles of the two-electron processes (src/rbeb.jl:65-76,
+// src/moller.jl:218-229, src/bhaba.jl:311-322)
+__device__ __forceinline__ void two_body(const V3& p, double E0, double E1, double E2, double phi,
+                                         V3& p1v, V3& p2v) {
+    double p1 = sqrt(E1 * E1 + 2.0 * MC2 * E1) / C;
+    double p2 = sqrt(E2 * E2 + 2.0 * MC2 * E2) / C;
+    double c1 = sqrt(E1 * (E0 + 2.0 * MC2) / (E0 * (E1 + 2.0 * MC2)));
+    double c2 = sqrt(E2 * (E0 + 2.0 * MC2) / (E0 * (E2 + 2.0 * MC2)));
+    p1v = turn(p, c1, phi, p1);
+    p2v = turn(p, c2, -phi, p2);
+}
+
+template <class G>
+__device__ void collide(const DevProc& pr, int sp, const V3& p, double eng, const DevSB* sbs,
+                        const DevPE* pes, G& g, Outcome& o) {
+    o.err = 0;
+    o.nnew = 0;
+    switch (pr.kind) {
+    case PT_PROC_ELASTIC: {  // src/relativistic_coulomb.jl:10-23
+        double phi = TWO_PI * g.next();
+        V3 v = velocity(sp, p);
+        double beta = norm3(v) / C;
+        double a = pr.c[0];
+        double al = HBAR * HBAR / (4.0 * dot3(p, p) * (a * a));
+        double cost = sample_rel_sr(al, beta, g);
+        o.p0 = turn(p, cost, phi, norm3(p));
+        o.s0 = nextcoll(g);
+        o.kind = OUT_STATECHANGE;
+        break;
+    }
+    case PT_PROC_RBEB:  // src/rbeb.jl:54-80
+    case PT_PROC_MOLLER:  // src/moller.jl:13-37
+    case PT_PROC_BHABHA: {  // src/bhaba.jl:9-33
+        double E0 = eng, E1, E2;
+        if (pr.kind == PT_PROC_RBEB) {
+            E2 = rbeb_sample(E0, pr.B, g);
+            E1 = E0 - E2 - pr.B;
+            if (!(E2 < E1)) o.err = 1;
+        } else {
+            E2 = pr.kind == PT_PROC_MOLLER ? moller_sample(pr.tcut, eng, g)
+                                           : bhabha_sample(pr.tcut, eng, g);
+            E1 = E0 - E2;
+        }
+        double phi = TWO_PI * g.next();
+        V3 p1v, p2v;
+        two_body(p, E0, E1, E2, phi, p1v, p2v);
+        o.kind = OUT_NEW;
+        o.p0 = p1v;
+        o.s0 = nextcoll(g);
+        o.q[0].p = p2v;
+        o.q[0].s = nextcoll(g);
+        o.q[0].sp = PT_ELECTRON;
+        o.nnew = 1;
+        break;
+    }
+    case PT_PROC_SB: {  // src/seltzer.jl:67-90
+        double kf = sb_sample(sbs[pr.slot], eng, g);
+        if (kf < 0.0) o.err = 4;
+        double k = eng * kf;
+        if (!(k < eng)) o.err = 2;
+        double pnorm = k / C;
+        double cost = sample_tsai(eng, g);
+        double phi = TWO_PI * g.next();
+        V3 pph = turn(p, cost, phi, pnorm);
+        o.kind = OUT_NEW;
+        o.p0 = V3{p.x - pph.x, p.y - pph.y, p.z - pph.z};
+        o.s0 = nextcoll(g);
+        o.q[0].p = pph;
+        o.q[0].s = nextcoll(g);
+        o.q[0].sp = PT_PHOTON;
+        o.nnew = 1;
+        break;
+    }
+    case PT_PROC_COMPTON:
+    case PT_PROC_KN_COMPTON: {  // src/compton.jl:9-28
+        double E1, cost;
+        compton_sample(eng, g, E1, cost);
+        double phi = TWO_PI * g.next();
+        V3 pg = turn(p, cost, phi, E1 / C);
+        o.kind = OUT_NEW;
+        o.p0 = pg;
+        o.s0 = nextcoll(g);
+        o.q[0].p = V3{p.x - pg.x, p.y - pg.y, p.z - pg.z};
+        o.q[0].s = nextcoll(g);
+        o.q[0].sp = PT_ELECTRON;
+        o.nnew = 1;
+        break;
+    }
+    case PT_PROC_PHOTOELECTRIC: {  // src/photo_electric.jl:38-52, :60-76
+        const DevPE& pe = pes[pr.slot];
+        double b = pe.binding[0];
+        for (int q = 0; q < pe.nshell; q++) {
+            b = pe.binding[q];
+            if (eng > b) break;
+        }
+        if (!(eng > b)) o.err = 3;
+        double ee = eng - b;
+        double cost = sauter_sample(ee, g);
+        double phi = TWO_PI * g.next();
+        double pn = mom_from_kin(PT_ELECTRON, ee);
+        o.kind = OUT_REPLACE;
+        o.q[0].p = turn(p, cost, phi, pn);
+        o.q[0].s = nextcoll(g);
+        o.q[0].sp = PT_ELECTRON;
+        o.nnew = 1;
+        break;
+    }
+    case PT_PROC_BETHE_HEITLER: {  // src/bethe_heitler.jl:5-25 (names swapped as there)
+        double pkin, ekin;
+        bh_sample(pr.c[0], pr.c[1], pr.c[2], eng, g, pkin, ekin);
+        double phi = TWO_PI * g.next();
+        double cost = sample_tsai(ekin, g);
+        double pn = mom_from_kin(PT_ELECTRON, ekin);
+        V3 pe = turn(p, cost, phi, pn);
+        cost = sample_tsai(pkin, g);
+        pn = mom_from_kin(PT_POSITRON, pkin);
+        V3 pp = turn(p, cost, phi, pn);
+        o.kind = OUT_REPLACE_PAIR;
+        o.q[0].p = pe;
+        o.q[0].s = nextcoll(g);
+        o.q[0].sp = PT_ELECTRON;
+        o.q[1].p = pp;
+        o.q[1].s = nextcoll(g);
+        o.q[1].sp = PT_POSITRON;
+        o.nnew = 2;
+        break;
+    }
+    case PT_PROC_ANNIHILATION: {  // src/anihilation.jl:6-23, :61-67
+        double eps = annih_sample(eng, g);
+        double gm = 1.0 + eng / MC2;
+        double cost = (eps * (gm + 1.0) - 1.0) / (eps * sqrt(gm * gm - 1.0));
+        double phi = TWO_PI * g.next();
+        double etot = eng + 2.0 * MC2;
+        double pan = eps * etot / C;
+        V3 pa = turn(p, cost, phi, pan);
+        o.kind = OUT_REPLACE_PAIR;
+        o.q[0].p = pa;
+        o.q[0].s = nextcoll(g);
+        o.q[0].sp = PT_PHOTON;
+        o.q[1].p = V3{p.x - pa.x, p.y - pa.y, p.z - pa.z};
+        o.q[1].s = nextcoll(g);
+        o.q[1].sp = PT_PHOTON;
+        o.nnew = 2;
+        break;
+    }
+    default:
+        o.kind = OUT_NULL;
+        break;
+    }
+}
+
+}  // namespace pt

@@ -1,0 +1,24 @@
+"""particulator_hip: Particulator.jl's null-collision Monte Carlo step on MI355X.
+
+Host mirror of the reference's API (populations, tables, fields, pushers, callbacks,
+run loop) over the C ABI of libparticulator_hip.so (include/particulator_hip.h).
+"""
+from . import constants
+from . import constants as co
+from ._lib import (PT_ELECTRON, PT_PHOTON, PT_POSITRON, PTError, lib)
+from .air import build_tables, composition
+from .callbacks import (AbstractCallback, CollisionCounter, CombinedCallback,
+                        ParticleCountCallback, PopulationTargetCallback, RouletteCallback,
+                        SplitCallback, VoidCallback)
+from .fields import (ConfinedDoubleLayerField, DoubleLayerField, ElectromagneticField,
+                     HomogeneousField, NullForcing, RK2Pusher, StepField)
+from .physics import (ORBITALS, RBEB, BetheHeitler, Bhaba, Bhabha, Compton, KleinNishinaCompton,
+                      Moller, NullCollision, PhotoElectric, PositronAnihilation,
+                      RelativisticCoulomb, SeltzerBerger)
+from .population import (ElectronState, MultiPopulation, PhotonState, Population,
+                         PositronState, advance_, droplow_, init_, repack_, roulette_, split_)
+from .run import run_
+from .tables import (BinaryIntervals, ChebyshevCollisionTable, chebfit,
+                     collision_table_from_processes)
+
+Electron, Photon, Positron = PT_ELECTRON, PT_PHOTON, PT_POSITRON

@@ -1,0 +1,146 @@
+"""ctypes binding of libparticulator_hip.so (include/particulator_hip.h).
+
+The library is built in-tree by build.py (hipcc for gfx950).  Loading it never falls back
+to anything else: if the shared object is missing, `lib()` raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBNAME = "libparticulator_hip.so"
+LIBPATH = os.path.join(HERE, LIBNAME)
+
+PT_ELECTRON, PT_PHOTON, PT_POSITRON = 0, 1, 2
+NSPECIES = 3
+(PROC_NULL, PROC_ELASTIC, PROC_SB, PROC_RBEB, PROC_MOLLER, PROC_COMPTON, PROC_PHOTOELECTRIC,
+ PROC_BETHE_HEITLER, PROC_BHABHA, PROC_ANNIHILATION, PROC_KN_COMPTON) = range(11)
+FIELD_NONE, FIELD_HOMOGENEOUS, FIELD_DOUBLELAYER, FIELD_STEP, FIELD_CONFINED = range(5)
+NFIELDS = 10
+FIELD_NAMES = ("x", "y", "z", "px", "py", "pz", "w", "t", "s", "r")
+
+STATUS = {
+    0: "PT_OK", -1: "PT_E_ARG", -2: "PT_E_HIP", -3: "PT_E_CAPACITY", -4: "PT_E_RATEBOUND",
+    -5: "PT_E_STATE", -6: "PT_E_NOMEM", -7: "PT_E_TABLE_RANGE", -8: "PT_E_SAMPLER",
+    -9: "PT_E_NOT_BUILT",
+}
+
+
+class PTError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (STATUS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class ProcDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("Z", C.c_int32), ("slot", C.c_int32), ("N", C.c_int32),
+                ("B", C.c_double), ("U", C.c_double), ("tcut", C.c_double), ("zf", C.c_double),
+                ("c", C.c_double * 4)]
+
+
+class TableDesc(C.Structure):
+    _fields_ = [("nprocs", C.c_int32), ("order", C.c_int32), ("nintervals", C.c_int32),
+                ("nswarm", C.c_int32), ("xmax", C.c_double), ("energy_cut", C.c_double),
+                ("rate", C.POINTER(C.c_double)), ("ratebound", C.POINTER(C.c_double)),
+                ("procs", C.POINTER(ProcDesc))]
+
+
+class FieldFn(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("v", C.c_double * 3),
+                ("v2", C.c_double * 3), ("z1", C.c_double), ("z2", C.c_double),
+                ("sx", C.c_double), ("sy", C.c_double), ("sz", C.c_double), ("ez0", C.c_double)]
+
+
+class FieldDesc(C.Structure):
+    _fields_ = [("e", FieldFn), ("b", FieldFn)]
+
+
+class Config(C.Structure):
+    _fields_ = [("capacity", C.c_uint64 * 3), ("seed", C.c_uint64), ("tracked", C.c_int32 * 3),
+                ("nswarm", C.c_int32), ("device", C.c_int32), ("tally_bins", C.c_int32),
+                ("tally_lo", C.c_double), ("tally_hi", C.c_double)]
+
+
+class PopStats(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("nactive", C.c_uint64), ("weight", C.c_double),
+                ("mean_energy", C.c_double), ("centroid", C.c_double * 3),
+                ("spread", C.c_double), ("max_energy", C.c_double)]
+
+
+class Work(C.Structure):
+    _fields_ = [("particle_steps", C.c_uint64), ("secondaries", C.c_uint64),
+                ("moved", C.c_uint64), ("events", C.c_uint64), ("steps", C.c_uint64),
+                ("tail_particles", C.c_uint64), ("advance_ms", C.c_double),
+                ("advance_launches", C.c_uint64), ("tail_ms", C.c_double),
+                ("compact_ms", C.c_double), ("main_secondaries", C.c_uint64)]
+
+
+DP = C.POINTER(C.c_double)
+SOA = DP * NFIELDS
+VP = C.c_void_p
+
+# name -> (restype, argtypes); the C ABI declared in include/particulator_hip.h
+SIGNATURES = {
+    "pt_abi_version": (C.c_int, []),
+    "pt_create": (C.c_int, [C.POINTER(Config), C.POINTER(VP)]),
+    "pt_destroy": (None, [VP]),
+    "pt_last_error": (C.c_char_p, [VP]),
+    "pt_upload_table": (C.c_int, [VP, C.c_int, C.POINTER(TableDesc)]),
+    "pt_upload_sb": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, C.c_int, DP, DP]),
+    "pt_upload_pe": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, DP]),
+    "pt_set_field": (C.c_int, [VP, C.c_int, C.POINTER(FieldDesc)]),
+    "pt_add_particles": (C.c_int, [VP, C.c_int, SOA, C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_int32), C.c_size_t]),
+    "pt_init_particles": (C.c_int, [VP, C.c_int, C.c_size_t, C.c_double, C.c_double, C.c_int,
+                                    C.c_int, C.c_uint64, C.c_double]),
+    "pt_download": (C.c_int, [VP, C.c_int, SOA, C.POINTER(C.c_uint8), C.POINTER(C.c_uint64),
+                              C.POINTER(C.c_int32), C.c_size_t, C.POINTER(C.c_size_t)]),
+    "pt_set_particles": (C.c_int, [VP, C.c_int, SOA, C.POINTER(C.c_uint8),
+                                   C.POINTER(C.c_uint64), C.POINTER(C.c_int32), C.c_size_t]),
+    "pt_count": (C.c_int, [VP, C.c_int, C.POINTER(C.c_uint64)]),
+    "pt_init": (C.c_int, [VP]),
+    "pt_advance": (C.c_int, [VP, C.c_double]),
+    "pt_run": (C.c_int, [VP, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]),
+    "pt_compact": (C.c_int, [VP, C.c_int]),
+    "pt_stats": (C.c_int, [VP, C.c_int, C.POINTER(PopStats)]),
+    "pt_collision_counts": (C.c_int, [VP, C.c_int, C.POINTER(C.c_int64), C.c_int, C.c_int]),
+    "pt_tally_read": (C.c_int, [VP, C.c_int, DP, C.c_int, DP, DP]),
+    "pt_work_read": (C.c_int, [VP, C.POINTER(Work), C.c_int]),
+    "pt_profile_enable": (C.c_int, [VP, C.c_int]),
+    "pt_set_step": (C.c_int, [VP, C.c_uint32]),
+    "pt_get_step": (C.c_int, [VP, C.POINTER(C.c_uint32)]),
+    "pt_check": (C.c_int, [VP, C.POINTER(C.c_int64)]),
+    "pt_synchronize": (C.c_int, [VP]),
+    "pt_debug_philox": (C.c_int, [VP, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                  C.POINTER(C.c_uint32), C.c_size_t]),
+    "pt_debug_uniforms": (C.c_int, [VP, C.c_uint64, C.c_uint32, DP, C.c_size_t]),
+    "pt_debug_select": (C.c_int, [VP, C.c_int, C.c_size_t, DP, DP, C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32), DP]),
+    "pt_debug_push": (C.c_int, [VP, C.c_int, C.c_size_t, SOA, DP]),
+    "pt_debug_collide": (C.c_int, [VP, C.c_int, C.c_int, C.c_size_t, SOA, C.POINTER(C.c_uint64),
+                                   DP, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32), DP,
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raise if it has not been built (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise PTError(-9, "%s is not built; run particulator_hip.build.build()" % LIBPATH)
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(ctx, rc):
+    if rc != 0:
+        msg = lib().pt_last_error(ctx).decode() if ctx else ""
+        raise PTError(rc, msg)
+    return rc

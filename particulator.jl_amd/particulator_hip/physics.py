@@ -1,0 +1,467 @@
+"""Collision processes of Particulator.jl: total cross sections (used at init to build the
+rate tables) and the descriptors the HIP kernels sample from.
+
+Each class mirrors the reference type of the same name; `totalcs(E)` is vectorised over a
+numpy array of kinetic energies (J) and evaluates the reference's formula in its order.
+The secondary samplers themselves run on the GPU (csrc/pt_device.h).
+"""
+import math
+import os
+
+import numpy as np
+
+from . import constants as co
+from ._lib import (PROC_ANNIHILATION, PROC_BETHE_HEITLER, PROC_BHABHA, PROC_COMPTON,
+                   PROC_ELASTIC, PROC_KN_COMPTON, PROC_MOLLER, PROC_PHOTOELECTRIC, PROC_RBEB,
+                   PROC_SB, ProcDesc)
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "data")
+
+
+def _arr(E):
+    return np.asarray(E, dtype=np.float64)
+
+
+def _jmax0(v):
+    """Julia max(0, v): NaN propagates."""
+    return np.where(np.isnan(v), v, np.maximum(v, 0.0))
+
+
+class CollisionProcess:
+    kind = 0
+    Z = 0
+
+    def desc(self, slots=None):
+        d = ProcDesc()
+        d.kind = self.kind
+        d.Z = int(self.Z)
+        d.zf = float(self.Z)
+        return d
+
+
+class RelativisticCoulomb(CollisionProcess):
+    """Screened relativistic Coulomb scattering, src/relativistic_coulomb.jl:5-75."""
+    kind = PROC_ELASTIC
+
+    def __init__(self, Z):
+        self.Z = Z
+
+    def screening_radius(self):
+        # a = 1.3413 * Z^(-1//3) * a_0 (src/relativistic_coulomb.jl:16)
+        return 1.3413 * float(self.Z) ** (-1.0 / 3.0) * co.a_0
+
+    def totalcs(self, K):
+        K = _arr(K) + 1e-4 * co.eV
+        Z = float(self.Z)
+        a = self.screening_radius()
+        with np.errstate(all="ignore"):
+            g = 1.0 + K / (co.electron_mass * co.c2)
+            p = np.sqrt(K * (K + 2.0 * (co.electron_mass * co.c2))) / co.c
+            b = p / (g * co.electron_mass * co.c)
+            al = co.hbar * co.hbar / (4.0 * (p * p) * (a * a))
+            b2 = b * b
+            return (co.pi * (co.r_e * co.r_e) * (Z * Z) / (b ** 4 * (g * g)) *
+                    ((1.0 + al * b2) / (al * (1.0 + al)) + b2 * np.log(al / (1.0 + al))))
+
+    def desc(self, slots=None):
+        d = super().desc()
+        d.c[0] = self.screening_radius()
+        return d
+
+
+class RBEB(CollisionProcess):
+    """Relativistic binary-encounter-Bethe ionisation of one orbital, src/rbeb.jl:5-150."""
+    kind = PROC_RBEB
+
+    def __init__(self, B, U, N):
+        self.B, self.U, self.N = B, U, N
+
+    def totalcs(self, T):
+        T = _arr(T)
+        B, U = self.B, self.U
+        mc2 = co.electron_mc2
+        with np.errstate(all="ignore"):
+            t1, b1, u1 = T / mc2, B / mc2, U / mc2
+            bt2 = 1.0 - 1.0 / ((1.0 + t1) * (1.0 + t1))
+            bb2 = 1.0 - 1.0 / ((1.0 + b1) * (1.0 + b1))
+            bu2 = 1.0 - 1.0 / ((1.0 + u1) * (1.0 + u1))
+            t = T / B
+            pre = 2.0 * (co.a_0 * co.a_0) * co.pi * co.alpha ** 4 / (b1 * ((bb2 + bt2) + bu2))
+            tt1 = (1.0 + t1) * (1.0 + t1)
+            v = 1.0 - 1.0 / t
+            v = v + (b1 * b1 * (t - 1.0)) / (2.0 * tt1)
+            v = v - ((1.0 + 2.0 * t1) * np.log(t)) / ((1.0 + t) * tt1)
+            v = v - ((t * t - 1.0) * ((bt2 - np.log(bt2)) + np.log(2.0 * b1 * (1.0 - bt2)))) / (2.0 * (t * t))
+            sig = float(self.N) * (pre * v)
+        return np.where(T > B, sig, 0.0)
+
+    def desc(self, slots=None):
+        d = super().desc()
+        d.B, d.U, d.N = self.B, self.U, self.N
+        return d
+
+
+# molecular orbitals (B, U in eV; N), src/rbeb.jl:25-48
+N2_ORBITALS = [RBEB(0.4095e3 * co.eV, 0.6033e3 * co.eV, 4), RBEB(41.72 * co.eV, 71.13 * co.eV, 2),
+               RBEB(21.00 * co.eV, 63.18 * co.eV, 2), RBEB(17.07 * co.eV, 44.30 * co.eV, 4),
+               RBEB(15.58 * co.eV, 54.91 * co.eV, 2)]
+O2_ORBITALS = [RBEB(0.5438e3 * co.eV, 0.7962e3 * co.eV, 4), RBEB(46.19 * co.eV, 79.73 * co.eV, 2),
+               RBEB(29.82 * co.eV, 90.92 * co.eV, 2), RBEB(19.64 * co.eV, 59.89 * co.eV, 4),
+               RBEB(19.79 * co.eV, 71.84 * co.eV, 2), RBEB(12.07 * co.eV, 84.88 * co.eV, 2)]
+ORBITALS = {"N2": N2_ORBITALS, "O2": O2_ORBITALS}
+
+
+class Moller(CollisionProcess):
+    """Moller ionisation above tcut, src/moller.jl:8-57."""
+    kind = PROC_MOLLER
+
+    def __init__(self, Z, tcut):
+        self.Z, self.tcut = Z, tcut
+
+    def totalcs(self, eng):
+        eng = _arr(eng)
+        Z = float(self.Z)
+        with np.errstate(all="ignore"):
+            x = self.tcut / eng
+            g = 1.0 + eng / co.electron_mc2
+            g2 = g * g
+            b2 = (g2 - 1.0) / g2
+            A = ((g - 1.0) * (g - 1.0) / g2) * (0.5 - x) + 1.0 / x
+            A = A - 1.0 / (1.0 - x)
+            A = A - ((2.0 * g - 1.0) / g2) * np.log((1.0 - x) / x)
+            v = _jmax0(co.two_pi * (co.r_e * co.r_e) * Z * A / (g - 1.0) / b2)
+        return np.where(x > 1.0, 0.0, v)
+
+    def desc(self, slots=None):
+        d = super().desc()
+        d.tcut = self.tcut
+        return d
+
+
+def _bhabha_bs(y):
+    q = 1.0 - 2.0 * y
+    return 2.0 - y * y, q * (3.0 + y * y), q * q + q * q * q, q * q * q
+
+
+class Bhaba(CollisionProcess):
+    """Bhabha scattering of positrons above tcut, src/bhaba.jl:4-49 (reference spelling)."""
+    kind = PROC_BHABHA
+
+    def __init__(self, Z, tcut):
+        self.Z, self.tcut = Z, tcut
+
+    def totalcs(self, eng):
+        eng = _arr(eng)
+        Z = float(self.Z)
+        with np.errstate(all="ignore"):
+            g = 1.0 + eng / co.electron_mc2
+            b = np.sqrt((g * g - 1.0) / (g * g))
+            x = self.tcut / eng
+            y = 1.0 / (g + 1.0)
+            B1, B2, B3, B4 = _bhabha_bs(y)
+            A = (1.0 / x - 1.0) / (b * b) + B1 * np.log(x)
+            A = A + B2 * (1.0 - x)
+            A = A - B3 * (1.0 - x * x) / 2.0
+            A = A + B4 * (1.0 - x * x * x) / 3.0
+            return _jmax0(co.two_pi * (co.r_e * co.r_e) * Z * A / (g - 1.0))
+
+    def desc(self, slots=None):
+        d = super().desc()
+        d.tcut = self.tcut
+        return d
+
+
+Bhabha = Bhaba
+
+
+class PositronAnihilation(CollisionProcess):
+    """Two-photon annihilation (Heitler), src/anihilation.jl:1-32 (reference spelling)."""
+    kind = PROC_ANNIHILATION
+
+    def __init__(self, Z):
+        self.Z = Z
+
+    def totalcs(self, eng):
+        eng = _arr(eng)
+        with np.errstate(all="ignore"):
+            g = 1.0 + eng / co.electron_mc2
+            g2 = g * g
+            sq = np.sqrt(g2 - 1.0)
+            A = ((g2 + 4.0 * g + 1.0) / (g2 - 1.0) * np.log(g + sq) - (g + 3.0) / sq) / (g + 1.0)
+            return float(self.Z) * co.pi * (co.r_e * co.r_e) * A
+
+
+PositronAnnihilation = PositronAnihilation
+
+
+class KleinNishinaCompton(CollisionProcess):
+    """Klein-Nishina cross section, src/compton.jl:34-47."""
+    kind = PROC_KN_COMPTON
+
+    def __init__(self, Z):
+        self.Z = Z
+
+    def totalcs(self, eng):
+        eng = _arr(eng)
+        with np.errstate(all="ignore"):
+            x = 2.0 * eng / co.electron_mc2
+            sigma = co.two_pi * (co.r_e * co.r_e) * (1.0 / x) * (
+                (((1.0 - 4.0 / x) - 8.0 / (x * x)) * np.log(1.0 + x) + 0.5) + 8.0 / x -
+                1.0 / ((1.0 + x) * (1.0 + x)) / 2.0)
+            return float(self.Z) * sigma
+
+
+class Compton(CollisionProcess):
+    """Geant4 empirical Compton cross section, src/compton.jl:59-112."""
+    kind = PROC_COMPTON
+
+    def __init__(self, Z):
+        self.Z = Z
+
+    def totalcs(self, eng):
+        eng = _arr(eng)
+        Z = float(self.Z)
+        barn = 1e-28
+        a, b, c = 20.0, 230.0, 440.0
+        d1, d2, d3, d4 = 2.7965e-1 * barn, -1.8300e-1 * barn, 6.7527 * barn, -1.9798e+1 * barn
+        e1, e2, e3, e4 = 1.9756e-5 * barn, -1.0205e-2 * barn, -7.3913e-2 * barn, 2.7079e-2 * barn
+        f1, f2, f3, f4 = -3.9178e-7 * barn, 6.8241e-5 * barn, 6.0480e-5 * barn, 3.0274e-4 * barn
+        Z2 = Z * Z
+        p1Z = Z * ((d1 + e1 * Z) + f1 * Z2)
+        p2Z = Z * ((d2 + e2 * Z) + f2 * Z2)
+        p3Z = Z * ((d3 + e3 * Z) + f3 * Z2)
+        p4Z = Z * ((d4 + e4 * Z) + f4 * Z2)
+        T0 = 15.0e3 * co.eV
+        mc2 = co.electron_mc2
+
+        def sig(X):
+            return (p1Z * np.log(1.0 + 2.0 * X) / X +
+                    ((p2Z + p3Z * X) + p4Z * (X * X)) / (((1.0 + a * X) + b * (X * X)) + c * (X * X * X)))
+
+        with np.errstate(all="ignore"):
+            X = np.maximum(eng, T0) / mc2
+            sigma = sig(X)
+            X1 = (T0 + 1e3 * co.eV) / mc2
+            sigma1 = sig(np.float64(X1))
+            c1 = -T0 * (sigma1 - sigma) / (sigma * (1e3 * co.eV))
+            c2 = 0.375 - 0.0556 * math.log(Z) if Z > 1.5 else 0.150
+            y = np.log(eng / T0)
+            low = sigma * np.exp(-y * (c1 + c2 * y))
+        return np.where(eng < T0, low, sigma)
+
+
+def _bh_fc():
+    """Coulomb correction _fc (src/bethe_heitler.jl:253-261); alpha, not alpha*Z, as there."""
+    az = co.alpha
+    az2 = az * az
+    az4 = az2 * az2
+    az6 = az4 * az2
+    f1 = (((1.0 / (1.0 + az2) + 0.20206) - 0.0369 * az2) + 0.0083 * az4) - 0.0020 * az6
+    return f1 * az2
+
+
+class BetheHeitler(CollisionProcess):
+    """Pair production, src/bethe_heitler.jl:1-80."""
+    kind = PROC_BETHE_HEITLER
+
+    def __init__(self, Z):
+        self.Z = Z
+
+    def totalcs(self, eng):
+        eng = _arr(eng)
+        Z = float(self.Z)
+        mb = 1e-34
+        lim = 1.5e6 * co.eV
+        A = [8.7842e+2, -1.9625e+3, 1.2949e+3, -2.0028e+2, 1.2575e+1, -2.8333e-1]
+        B = [-1.0342e+1, 1.7692e+1, -8.2381, 1.3063, -9.0815e-2, 2.3586e-3]
+        Cc = [-4.5263e+2, 1.1161e+3, -8.6749e+2, 2.1773e+2, -2.0467e+1, 6.5372e-1]
+        with np.errstate(all="ignore"):
+            x = np.log(np.maximum(eng, lim) / co.electron_mc2)
+            xs = [None, x, x * x]
+            xs.append(xs[2] * x)
+            xs.append(xs[3] * x)
+            xs.append(xs[4] * x)
+
+            def poly(cf):
+                acc = cf[0] * mb + (cf[1] * mb) * xs[1]
+                for q in range(2, 6):
+                    acc = acc + (cf[q] * mb) * xs[q]
+                return acc
+
+            F1, F2, F3 = poly(A), poly(B), poly(Cc)
+            sigma = (Z + 1.0) * ((F1 * Z + F2 * Z * Z) + F3)
+            q = (eng - 2.0 * co.electron_mc2) / (lim - 2.0 * co.electron_mc2)
+            sigma = np.where(eng < lim, sigma * (q * q), sigma)
+        return np.where(eng < 2.0 * co.electron_mc2, 0.0, sigma)
+
+    def desc(self, slots=None):
+        d = super().desc()
+        fz = 8.0 * math.log(float(self.Z)) / 3.0
+        d.c[0] = float(self.Z) ** (1.0 / 3.0)
+        d.c[1] = fz
+        d.c[2] = fz + 8.0 * _bh_fc()
+        return d
+
+
+# ---- photoelectric: Sandia rows (keV, cm^2 keV^i / g) and binding energies (eV)
+# N: src/static_sandia_data.jl:134-143, O: :145-156, Ar: :254-265; Z/A :1405-1441;
+# shells src/atomic_shells.jl:37, :40, :70
+SANDIA = {
+    7: [(0.01, 0.1010E+05, 0.0000E+00, 0.0000E+00, 0.0000E+00),
+        (0.0404, -0.3622E+03, 0.3873E+03, 0.1244E+02, -0.4452E+00),
+        (0.4016, -0.2338E+04, 0.5732E+04, -0.2082E+03, 0.1482E+03),
+        (0.8, -0.4940E+01, -0.8442E+02, 0.4620E+04, -0.1186E+04),
+        (4.0, 0.2019E+01, -0.1249E+03, 0.4609E+04, -0.9421E+03),
+        (20.0, 0.1709E-01, -0.8196E+01, 0.2345E+04, 0.1369E+05),
+        (100.0, 0.1872E-02, -0.6732E+00, 0.1282E+04, 0.5700E+05),
+        (500.0, 0.8122E-03, 0.8364E+00, 0.4410E+03, 0.2358E+06)],
+    8: [(0.01, 9.343E+03, 9.026E+02, -2.467E+01, 1.505E-01),
+        (0.02, 6.034E+03, 7.319E+02, -2.677E+01, 2.842E-01),
+        (0.0483, -0.2863E+03, 0.4085E+03, 0.4436E+02, -0.1782E+01),
+        (0.532, -0.7181E+02, 0.4748E+03, 0.5542E+04, -0.1363E+04),
+        (4.0, 0.2745E+01, -0.1747E+03, 0.7159E+04, -0.2213E+04),
+        (20.0, 0.3774E-01, -0.1559E+02, 0.4045E+04, 0.1810E+05),
+        (100.0, 0.3169E-02, -0.1146E+01, 0.2194E+04, 0.9131E+05),
+        (500.0, 0.1367E-02, 0.1473E+01, 0.7214E+03, 0.4048E+06)],
+    18: [(0.01, -8.294E+04, 5.720E+03, -1.101E+02, 6.668E-01),
+         (0.04, -1.401E+03, 9.577E+02, -7.842E+01, 1.797E+00),
+         (0.245, -0.2244E+03, 0.1388E+04, 0.2571E+04, -0.5113E+03),
+         (3.203, -0.1446E+02, 0.4359E+03, 0.6578E+05, -0.7284E+05),
+         (20.0, 0.8786E+00, -0.3001E+03, 0.7682E+05, -0.1219E+06),
+         (100.0, 0.6779E-01, -0.2695E+02, 0.5069E+05, 0.5281E+06),
+         (500.0, 0.2598E-01, 0.3454E+02, 0.1059E+05, 0.1068E+08)],
+}
+Z_TO_A = {7: 0.4998, 8: 0.5, 18: 0.4506}
+SHELLS = {7: [403.0, 37.3, 20.33, 14.53], 8: [543.1, 41.6, 28.48, 13.62],
+          18: [3206.3, 326.5, 250.6, 248.5, 29.24, 15.94, 15.76]}
+
+
+class PhotoElectric(CollisionProcess):
+    """Photoelectric absorption with Sandia cross sections, src/photo_electric.jl:7-58."""
+    kind = PROC_PHOTOELECTRIC
+
+    def __init__(self, Z):
+        if Z not in SANDIA:
+            raise ValueError("photoelectric data are included for Z in %s" % sorted(SANDIA))
+        self.Z = Z
+        rows = SANDIA[Z]
+        A = Z / Z_TO_A[Z]
+        kev = co.kilo * co.eV
+        s = [kev, kev ** 2, kev ** 3, kev ** 4]
+        conv = co.centi * co.centi * A / co.N_A
+        self.left_energy = np.array([r[0] * kev for r in rows])
+        self.coeffs = np.array([[r[i + 1] * s[i] * conv for i in range(4)] for r in rows])  # [nint, 4]
+        self.binding = np.array(SHELLS[Z]) * co.eV
+
+    def totalcs(self, eng):
+        eng = _arr(eng)
+        j = np.searchsorted(self.left_energy, eng, side="right") - 1
+        jj = np.clip(j, 0, len(self.left_energy) - 1)
+        with np.errstate(all="ignore"):
+            e1 = 1.0 / eng
+            ep = [e1, e1 * e1, eng ** -3.0, eng ** -4.0]
+            acc = ep[0] * self.coeffs[jj, 0]
+            for i in range(1, 4):
+                acc = acc + ep[i] * self.coeffs[jj, i]
+        return np.where(j >= 0, acc, 0.0)
+
+    def desc(self, slots=None):
+        d = super().desc()
+        d.slot = slots.pe_slot(self) if slots is not None else 0
+        return d
+
+
+# ---- Seltzer-Berger
+def read_g4_2d(path):
+    """RawG4Physics2DVector (src/util.jl:165-190): `k nx ny`, x, y, then ny x nx values."""
+    with open(path) as f:
+        k, nx, ny = (int(v) for v in f.readline().split())
+        x = np.array([float(v) for v in f.readline().split()])
+        y = np.array([float(v) for v in f.readline().split()])
+        vals = np.array([float(v) for v in f.read().split()]).reshape(ny, nx)
+    if x.size != nx or y.size != ny:
+        raise ValueError("malformed G4 2D vector %s" % path)
+    return x, y, vals
+
+
+def _pl_cum(x, p):
+    """cumulative integral of the piecewise-linear interpolant at the knots"""
+    seg = (p[:-1] + p[1:]) * np.diff(x) / 2.0
+    return np.concatenate([[0.0], np.cumsum(seg)])
+
+
+def _pl_cum_at(x, p, F, xq):
+    m = np.clip(np.searchsorted(x, xq, side="right") - 1, 0, len(x) - 2)
+    d = xq - x[m]
+    slope = (p[m + 1] - p[m]) / (x[m + 1] - x[m])
+    return F[m] + p[m] * d + slope * d * d / 2.0
+
+
+def sb_inverse_cdf(logk, chi, pcum, xmin, xmax):
+    """Quantiles of the normalised cumulative of the piecewise-linear interpolant of chi
+    over logk (src/seltzer.jl:163-192), solved in closed form: on each knot interval the
+    cumulative is quadratic."""
+    F = _pl_cum(logk, chi)
+    lo, hi = max(xmin, logk[0]), min(xmax, logk[-1])
+    c0, c1 = _pl_cum_at(logk, chi, F, lo), _pl_cum_at(logk, chi, F, hi)
+    target = c0 + pcum * (c1 - c0)
+    m = np.clip(np.searchsorted(F, target, side="right") - 1, 0, len(logk) - 2)
+    h = logk[m + 1] - logk[m]
+    a = (chi[m + 1] - chi[m]) / (2.0 * h)
+    b = chi[m]
+    cc = target - F[m]
+    d = 2.0 * cc / (b + np.sqrt(b * b + 4.0 * a * cc))
+    out = logk[m] + d
+    out = np.clip(out, lo, hi)
+    out[0], out[-1] = lo, hi
+    return out, c1 - c0
+
+
+class SeltzerBerger(CollisionProcess):
+    """Seltzer-Berger bremsstrahlung, src/seltzer.jl:9-64, :217-231."""
+    kind = PROC_SB
+
+    def __init__(self, Z, ncum=1000, gamma_min=1e2 * co.eV, gamma_max=5e7 * co.eV,
+                 energy_scale=1e6 * co.eV, path=None):
+        self.Z = Z
+        path = path or os.path.join(DATA_DIR, "brem_SB", "br%d" % Z)
+        x, y, vals = read_g4_2d(path)
+        self.log_energy = np.log(np.exp(y) * energy_scale)
+        self.pcum = np.arange(ncum, dtype=np.float64) / (ncum - 1)
+        ne = y.size
+        self.data = np.zeros((ne, ncum))
+        self.tcs = np.zeros(ne)
+        logk = np.log(x)
+        mc2 = co.electron_mc2
+        for i in range(ne):
+            T1 = math.exp(self.log_energy[i])
+            r = 1.0 + T1 / mc2
+            beta = math.sqrt(1.0 - 1.0 / (r * r))
+            q, cs = sb_inverse_cdf(logk, vals[i], self.pcum, math.log(gamma_min / T1),
+                                   math.log(gamma_max / T1))
+            self.data[i] = q
+            self.tcs[i] = (float(Z * Z) / (beta * beta)) * 1e-31 * cs
+
+    def totalcs(self, K):
+        K = _arr(K)
+        with np.errstate(all="ignore"):
+            lk = np.log(K)
+        le = self.log_energy
+        i = np.searchsorted(le, lk, side="right") - 1
+        ii = np.clip(i, 0, len(le) - 2)
+        w = (le[ii + 1] - lk) / (le[ii + 1] - le[ii])
+        v = w * self.tcs[ii] + (1.0 - w) * self.tcs[ii + 1]
+        return np.where(i >= 0, v, 0.0)
+
+    def desc(self, slots=None):
+        d = super().desc()
+        d.slot = slots.sb_slot(self) if slots is not None else 0
+        return d
+
+
+class NullCollision(CollisionProcess):
+    """src/collisions.jl:3"""
+    kind = 0
+
+    def totalcs(self, E):
+        return np.zeros_like(_arr(E))

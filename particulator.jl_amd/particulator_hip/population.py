@@ -1,0 +1,420 @@
+"""Populations, the multi-species population, the step and the run loop of Particulator.jl
+(src/population.jl, src/mixed_population.jl, src/run.jl), backed by device memory.
+
+A `MultiPopulation` owns one library context (`pt_ctx`) holding every species' particles
+in HBM.  `advance` is one pt_advance call (advance! + the cut of droplow!); `repack` /
+`droplow` free the slots of inactive particles (pt_compact).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import constants as co
+from ._lib import (FIELD_NAMES, NFIELDS, PT_ELECTRON, PT_PHOTON, PT_POSITRON, SOA, Config,
+                   PopStats, PTError, Work, check, lib)
+
+SPECIES = {"electron": PT_ELECTRON, "photon": PT_PHOTON, "positron": PT_POSITRON}
+SPECIES_NAME = {v: k for k, v in SPECIES.items()}
+
+STATE_DTYPE = np.dtype([("x", "f8", 3), ("p", "f8", 3), ("w", "f8"), ("t", "f8"), ("s", "f8"),
+                        ("r", "f8"), ("active", "?"), ("uid", "u8"), ("swarm", "i4")])
+
+_host_rng = np.random.default_rng(0)
+
+
+def nextcoll(rng=None):
+    """-log(rand()), src/util.jl:17"""
+    g = rng or _host_rng
+    return -math.log(1.0 - g.random())
+
+
+class _State:
+    species = None
+
+    def __init__(self, x, p, w=1.0, t=0.0, s=None, r=0.0, active=True, rng=None):
+        self.x = np.asarray(x, dtype=np.float64)
+        self.p = np.asarray(p, dtype=np.float64)
+        self.w, self.t = float(w), float(t)
+        self.s = nextcoll(rng) if s is None else float(s)
+        self.r, self.active = float(r), bool(active)
+
+    def kinenergy(self):
+        return kinenergy(self.species, self.p)
+
+
+class ElectronState(_State):
+    """src/electron.jl:13-34"""
+    species = PT_ELECTRON
+
+
+class PhotonState(_State):
+    """src/photon.jl:6-33"""
+    species = PT_PHOTON
+
+
+class PositronState(_State):
+    """src/positron.jl:3-22"""
+    species = PT_POSITRON
+
+
+def kinenergy(species, p):
+    p = np.asarray(p, dtype=np.float64)
+    pp = (p[..., 0] * p[..., 0] + p[..., 1] * p[..., 1]) + p[..., 2] * p[..., 2]
+    if species == PT_PHOTON:
+        return np.sqrt(pp) * co.c
+    return np.sqrt(co.electron_mc2 * co.electron_mc2 + co.c2 * pp) - co.electron_mc2
+
+
+def momentum_norm_from_kin(species, kin):
+    if species == PT_PHOTON:
+        return kin / co.c
+    a = kin + co.electron_mc2
+    return math.sqrt(a * a - co.electron_mc2 * co.electron_mc2) / co.c
+
+
+def states_to_soa(states):
+    n = len(states)
+    soa = np.zeros((NFIELDS, n))
+    for i, s in enumerate(states):
+        soa[0:3, i] = s.x
+        soa[3:6, i] = s.p
+        soa[6, i], soa[7, i], soa[8, i], soa[9, i] = s.w, s.t, s.s, s.r
+    return soa
+
+
+def _soa_ptrs(soa):
+    arr = SOA()
+    for f in range(NFIELDS):
+        arr[f] = soa[f].ctypes.data_as(C.POINTER(C.c_double)) if soa[f] is not None else None
+    return arr
+
+
+class Population:
+    """Population(max_particles, inparticles, collisions, energy_cut), src/population.jl:34-44.
+
+    A description until it joins a MultiPopulation, which allocates its device slots."""
+
+    def __init__(self, max_particles, inparticles, collisions, energy_cut=0.0, species=None,
+                 tracked=True):
+        self.max_particles = int(max_particles)
+        self.collisions = collisions
+        self.energy_cut = float(energy_cut)
+        if species is None:
+            species = collisions.species if collisions is not None else (
+                inparticles[0].species if len(inparticles) else None)
+        self.species = species
+        self.tracked = tracked
+        self.initial = list(inparticles)
+        self.mp = None
+
+    # statistics (src/population.jl:77-223), evaluated on the device
+    def _stats(self):
+        return self.mp.stats(self.species)
+
+    def nparticles(self):
+        return self.mp.count(self.species)
+
+    def nactives(self):
+        return int(self._stats().nactive)
+
+    def weight(self):
+        return self._stats().weight
+
+    def meanenergy(self):
+        return self._stats().mean_energy
+
+    def maxenergy(self):
+        return self._stats().max_energy
+
+    def spread(self):
+        s = self._stats()
+        return np.array(s.centroid[:]), s.spread
+
+    def particles(self, active_only=True):
+        return self.mp.download(self.species, active_only)
+
+
+class SlotMap:
+    """Assigns device data slots to Seltzer-Berger and photoelectric processes."""
+
+    def __init__(self):
+        self.sb, self.pe = [], []
+
+    def sb_slot(self, proc):
+        for i, p in enumerate(self.sb):
+            if p is proc:
+                return i
+        self.sb.append(proc)
+        return len(self.sb) - 1
+
+    def pe_slot(self, proc):
+        for i, p in enumerate(self.pe):
+            if p is proc:
+                return i
+        self.pe.append(proc)
+        return len(self.pe) - 1
+
+
+class MultiPopulation:
+    """MultiPopulation(:electron => ..., :photon => ..., :positron => ...),
+    src/mixed_population.jl:4-18, with all species resident on one GPU."""
+
+    def __init__(self, seed=0, device=0, nswarm=1, tally_bins=512, tally_range=(1e4 * co.eV, 1e7 * co.eV),
+                 **pops):
+        self.index = {}
+        for name, p in pops.items():
+            if name not in SPECIES:
+                raise ValueError("unknown species %r" % name)
+            p.species = SPECIES[name]
+            self.index[name] = p
+        cfg = Config()
+        for name, p in self.index.items():
+            cfg.capacity[p.species] = p.max_particles if p.tracked else 0
+            cfg.tracked[p.species] = 1 if p.tracked else 0
+        cfg.seed = seed
+        cfg.nswarm = nswarm
+        cfg.device = device
+        cfg.tally_bins = tally_bins
+        cfg.tally_lo, cfg.tally_hi = tally_range
+        self.cfg = cfg
+        h = C.c_void_p()
+        rc = lib().pt_create(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            msg = lib().pt_last_error(h).decode() if h.value else "pt_create failed"
+            if h.value:
+                lib().pt_destroy(h)
+            raise PTError(rc, msg)
+        self.ctx = h
+        self.slots = SlotMap()
+        self._keep = []
+        for name, p in self.index.items():
+            p.mp = self
+            if p.collisions is not None:
+                self.upload_table(p.species, p.collisions, p.energy_cut)
+        for name, p in self.index.items():
+            if p.initial:
+                self.add_states(p.species, p.initial)
+        self.pusher = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "ctx", None) is not None and self.ctx.value:
+                lib().pt_destroy(self.ctx)
+                self.ctx = None
+        except Exception:
+            pass
+
+    def close(self):
+        self.__del__()
+
+    def get(self, name):
+        return self.index[name]
+
+    def __getitem__(self, name):
+        return self.index[name]
+
+    # ---- tables, fields
+    def upload_table(self, species, table, energy_cut):
+        # data slots of SB / photoelectric processes
+        for p in table.proc:
+            if p.kind == 2:
+                slot = self.slots.sb_slot(p)
+                le = np.ascontiguousarray(p.log_energy)
+                data = np.ascontiguousarray(p.data)
+                self._keep += [le, data]
+                check(self.ctx, lib().pt_upload_sb(self.ctx, slot, int(p.Z), le.size, data.shape[1],
+                                                   le.ctypes.data_as(C.POINTER(C.c_double)),
+                                                   data.ctypes.data_as(C.POINTER(C.c_double))))
+            elif p.kind == 6:
+                slot = self.slots.pe_slot(p)
+                b = np.ascontiguousarray(p.binding)
+                self._keep.append(b)
+                check(self.ctx, lib().pt_upload_pe(self.ctx, slot, int(p.Z), b.size,
+                                                   b.ctypes.data_as(C.POINTER(C.c_double))))
+        d = table.desc(self.slots, energy_cut)
+        check(self.ctx, lib().pt_upload_table(self.ctx, species, C.byref(d)))
+
+    def set_pusher(self, pusher, swarm=0):
+        d = pusher.desc()
+        check(self.ctx, lib().pt_set_field(self.ctx, swarm, C.byref(d)))
+        if swarm == 0:
+            self.pusher = pusher
+
+    # ---- particles
+    def add_states(self, species, states, swarm=None):
+        soa = states_to_soa(states)
+        self.add_soa(species, soa, swarm=swarm)
+
+    def add_soa(self, species, soa, uid=None, swarm=None):
+        soa = np.ascontiguousarray(soa, dtype=np.float64)
+        n = soa.shape[1]
+        u = None if uid is None else np.ascontiguousarray(uid, dtype=np.uint64)
+        s = None if swarm is None else np.ascontiguousarray(np.broadcast_to(swarm, (n,)), dtype=np.int32)
+        check(self.ctx, lib().pt_add_particles(
+            self.ctx, species, _soa_ptrs(soa),
+            u.ctypes.data_as(C.POINTER(C.c_uint64)) if u is not None else None,
+            s.ctypes.data_as(C.POINTER(C.c_int32)) if s is not None else None, n))
+
+    def init_particles(self, species, n, e1, e2=None, loguniform=False, swarm=0, uid0=0, t0=0.0):
+        check(self.ctx, lib().pt_init_particles(self.ctx, species, n, e1, e2 if e2 else e1,
+                                                int(loguniform), swarm, uid0, t0))
+
+    def count(self, species):
+        n = C.c_uint64()
+        check(self.ctx, lib().pt_count(self.ctx, species, C.byref(n)))
+        return n.value
+
+    def download(self, species, active_only=True):
+        n = self.count(species)
+        soa = np.zeros((NFIELDS, max(n, 1)))
+        act = np.zeros(max(n, 1), dtype=np.uint8)
+        uid = np.zeros(max(n, 1), dtype=np.uint64)
+        sw = np.zeros(max(n, 1), dtype=np.int32)
+        got = C.c_size_t()
+        check(self.ctx, lib().pt_download(self.ctx, species, _soa_ptrs(soa),
+                                          act.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                          uid.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                          sw.ctypes.data_as(C.POINTER(C.c_int32)), soa.shape[1],
+                                          C.byref(got)))
+        n = got.value
+        out = np.zeros(n, dtype=STATE_DTYPE)
+        out["x"] = soa[0:3, :n].T
+        out["p"] = soa[3:6, :n].T
+        for f, name in ((6, "w"), (7, "t"), (8, "s"), (9, "r")):
+            out[name] = soa[f, :n]
+        out["active"] = act[:n] != 0
+        out["uid"] = uid[:n]
+        out["swarm"] = sw[:n]
+        return out[out["active"]] if active_only else out
+
+    def set_particles(self, species, arr):
+        """Replace a population with the states in a STATE_DTYPE array."""
+        n = arr.size
+        soa = np.zeros((NFIELDS, max(n, 1)))
+        soa[0:3, :n] = arr["x"].T
+        soa[3:6, :n] = arr["p"].T
+        for f, name in ((6, "w"), (7, "t"), (8, "s"), (9, "r")):
+            soa[f, :n] = arr[name]
+        act = np.ascontiguousarray(arr["active"].astype(np.uint8))
+        uid = np.ascontiguousarray(arr["uid"].astype(np.uint64))
+        sw = np.ascontiguousarray(arr["swarm"].astype(np.int32))
+        check(self.ctx, lib().pt_set_particles(self.ctx, species, _soa_ptrs(soa),
+                                               act.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                               uid.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                               sw.ctypes.data_as(C.POINTER(C.c_int32)), n))
+
+    # ---- step
+    def init(self):
+        """init!, src/mixed_population.jl:20-35"""
+        check(self.ctx, lib().pt_init(self.ctx))
+
+    def advance(self, tfinal):
+        check(self.ctx, lib().pt_advance(self.ctx, tfinal))
+
+    def run_steps(self, t0, dt, nsteps, compact_every=8, check_every=16):
+        check(self.ctx, lib().pt_run(self.ctx, t0, dt, nsteps, compact_every, check_every))
+        return t0 + nsteps * dt
+
+    def compact(self, species=-1):
+        check(self.ctx, lib().pt_compact(self.ctx, species))
+
+    def stats(self, species):
+        s = PopStats()
+        check(self.ctx, lib().pt_stats(self.ctx, species, C.byref(s)))
+        return s
+
+    def collision_counts(self, species, reset=False):
+        out = (C.c_int64 * 25)()
+        check(self.ctx, lib().pt_collision_counts(self.ctx, species, out, 25, int(reset)))
+        return np.array(out[:])
+
+    def tally(self, species):
+        nb = self.cfg.tally_bins
+        h = np.zeros(max(nb, 1))
+        cnt, w = C.c_double(), C.c_double()
+        check(self.ctx, lib().pt_tally_read(self.ctx, species, h.ctypes.data_as(C.POINTER(C.c_double)),
+                                            nb, C.byref(cnt), C.byref(w)))
+        return h[:nb], cnt.value, w.value
+
+    def work(self, reset=False):
+        w = Work()
+        check(self.ctx, lib().pt_work_read(self.ctx, C.byref(w), int(reset)))
+        return w
+
+    def profile(self, on=True):
+        check(self.ctx, lib().pt_profile_enable(self.ctx, int(on)))
+
+    @property
+    def step(self):
+        s = C.c_uint32()
+        check(self.ctx, lib().pt_get_step(self.ctx, C.byref(s)))
+        return s.value
+
+    @step.setter
+    def step(self, v):
+        check(self.ctx, lib().pt_set_step(self.ctx, v))
+
+    def flags(self):
+        f = (C.c_int64 * 5)()
+        rc = lib().pt_check(self.ctx, f)
+        return rc, list(f)
+
+
+# ---- reference free functions ------------------------------------------------------------
+def init_(mpopl):
+    """init!(mpopl), src/mixed_population.jl:20-35"""
+    mpopl.init()
+
+
+def advance_(mpopl, pusher, tfinal, callback=None):
+    """advance!(mpopl, pusher, tfinal, callback), src/mixed_population.jl:38-47."""
+    if pusher is not mpopl.pusher:
+        mpopl.set_pusher(pusher)
+    mpopl.advance(tfinal)
+
+
+def repack_(popl):
+    """repack!, src/population.jl:229-259"""
+    popl.mp.compact(popl.species)
+
+
+def droplow_(popl, thres=0.0):
+    """droplow!, src/population.jl:273-284.  The energy cut is applied inside every step;
+    this frees the slots (repack!)."""
+    if thres not in (0.0, popl.energy_cut):
+        raise ValueError("only the population's energy cut is supported on the device")
+    popl.mp.compact(popl.species)
+
+
+def roulette_(f, popl, rng=None):
+    """roulette!(f, popl), src/population.jl:291-309 (host-side, on downloaded states)."""
+    g = rng or _host_rng
+    arr = popl.mp.download(popl.species, active_only=False)
+    act = arr["active"]
+    if callable(f):
+        eng = kinenergy(popl.species, arr["p"])
+        prob = np.array([f(e) for e in eng])
+    else:
+        prob = np.full(arr.size, float(f))
+    u = g.random(arr.size)
+    keep = act & (u < prob)
+    arr["w"] = np.where(keep, arr["w"] / np.where(prob > 0, prob, 1.0), arr["w"])
+    arr["active"] = keep
+    popl.mp.set_particles(popl.species, arr)
+
+
+def split_(f, popl, rng=None):
+    """split!(f, popl), src/population.jl:316-330 (host-side)."""
+    g = rng or _host_rng
+    arr = popl.mp.download(popl.species, active_only=True)
+    if arr.size == 0:
+        return
+    p = np.array([f(a) for a in arr]) if callable(f) else np.full(arr.size, float(f))
+    arr["w"] = arr["w"] / (1.0 + p)
+    k = g.poisson(p)
+    extra = np.repeat(arr, k)
+    if extra.size:
+        base = int(arr["uid"].max()) + 1 if arr.size else 0
+        extra["uid"] = base + np.arange(extra.size, dtype=np.uint64) + (1 << 62)
+    popl.mp.set_particles(popl.species, np.concatenate([arr, extra]))

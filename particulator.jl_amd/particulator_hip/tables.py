@@ -1,0 +1,198 @@
+"""Binary-interval Chebyshev rate tables (src/cheby.jl) and the table builder
+collision_table_from_processes (src/collision_table.jl:115-167).
+
+Layout: `rate[i, j, o]` is the reference's `rate[o, j, i]` (Julia column-major (order,
+process, interval)), so the flat C array is the same bytes the reference holds.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import constants as co
+from ._lib import PT_PHOTON, ProcDesc, TableDesc
+
+DEFAULT_MAX_ENERGY = 1e3 * co.eV * 2 ** 18  # src/collision_table.jl:122
+
+
+class BinaryIntervals:
+    """Intervals 0..k of (0, xmax), src/cheby.jl:15-36."""
+
+    def __init__(self, k, xmax):
+        self.k, self.xmax = k, xmax
+
+    def interval(self, i):
+        if i == 0:
+            l1, _ = self.interval(1)
+            return 0.0, l1
+        lo = math.ldexp(1.0, i - self.k - 1)
+        return lo * self.xmax, 2.0 * lo * self.xmax
+
+    def precheb(self, x):
+        """(interval index, (T0, T1, T2)) of src/cheby.jl:127-143, vectorised."""
+        x = np.asarray(x, dtype=np.float64)
+        x1 = x / self.xmax
+        s, l = np.frexp(x1)
+        i = np.where(x1 == 0.0, 0, l + self.k)
+        xi = np.where(i > 0, 4.0 * s - 3.0, math.ldexp(1.0, self.k + 1) * x1 - 1.0)
+        i = np.where(i > 0, i, 0)
+        return i, (np.ones_like(xi), xi, 2.0 * xi * xi - 1.0)
+
+
+def chebnodes(n, a, b):
+    """src/cheby.jl:41-43"""
+    k = np.arange(n)
+    return (a + b) / 2.0 + (b - a) / 2.0 * np.cos((2.0 * k + 1.0) * math.pi / (2.0 * n))
+
+
+def chebval(xi, n):
+    t = [np.ones_like(xi), xi]
+    for _ in range(2, n):
+        t.append(2.0 * xi * t[-1] - t[-2])
+    return t[:n]
+
+
+def chebfit(f, b, n):
+    """Coefficients [k+1, n] of f on each interval, src/cheby.jl:211-227 (A \\ fx)."""
+    out = np.zeros((b.k + 1, n))
+    for i in range(b.k + 1):
+        lo, hi = b.interval(i)
+        x = chebnodes(n, lo, hi)
+        fx = np.asarray(f(x), dtype=np.float64)
+        xi = (2.0 * x - (lo + hi)) / (hi - lo)
+        A = np.stack(chebval(xi, n), axis=1)
+        out[i] = np.linalg.solve(A, fx)
+    return out
+
+
+def chebeval(x, b, a):
+    """sum_j a[i, j] T_j(xi) (src/cheby.jl:111-119); a is [k+1, n]"""
+    i, t = b.precheb(x)
+    ii = np.minimum(i, b.k)
+    acc = a[ii, 0] * t[0]
+    for o in range(1, a.shape[1]):
+        acc = acc + a[ii, o] * t[o]
+    return acc
+
+
+def chebdiff(x, b, a):
+    """derivative of the expansion (src/cheby.jl:180-203)"""
+    i, t = b.precheb(x)
+    ii = np.minimum(i, b.k)
+    xi = t[1]
+    lo = np.array([b.interval(q)[0] for q in range(b.k + 1)])
+    hi = np.array([b.interval(q)[1] for q in range(b.k + 1)])
+    u0, u1 = np.ones_like(xi), 2.0 * xi
+    df = a[ii, 1]
+    for q in range(3, a.shape[1] + 1):
+        df = df + (q - 1) * u1 * a[ii, q - 1]
+        u0, u1 = u1, 2.0 * xi * u1 - u0
+    return 2.0 * df / (hi[ii] - lo[ii])
+
+
+def speed(species, eng):
+    """speed(::Type{Electron|Positron|Photon}, eng), src/electron.jl:47, src/photon.jl:46"""
+    eng = np.asarray(eng, dtype=np.float64)
+    if species == PT_PHOTON:
+        return np.full_like(eng, co.c)
+    q = co.electron_mc2 / (co.electron_mc2 + eng)
+    return co.c * np.sqrt(1.0 - q * q)
+
+
+class ChebyshevCollisionTable:
+    """src/collision_table.jl:290-333.  `procs` are in table order."""
+
+    def __init__(self, procs, b, rate, ratebound, species, energy_cut=0.0, sort_keys=None):
+        self.proc = tuple(procs)
+        self.b = b
+        self.rate = np.ascontiguousarray(rate, dtype=np.float64)  # [k+1, L, 3]
+        rb = np.asarray(ratebound, dtype=np.float64)
+        self.ratebound = np.ascontiguousarray(rb.reshape(-1, b.k + 1, rb.shape[-1]))
+        self.species = species
+        self.energy_cut = energy_cut
+        self.sort_keys = sort_keys
+
+    def __len__(self):
+        return len(self.proc)
+
+    def rate_of(self, j, E):
+        i, t = self.b.precheb(E)
+        c = self.rate[np.minimum(i, self.b.k), j]
+        return (c[..., 0] * t[0] + c[..., 1] * t[1]) + c[..., 2] * t[2]
+
+    def ratebound_of(self, E, swarm=0):
+        i, t = self.b.precheb(E)
+        c = self.ratebound[swarm][np.minimum(i, self.b.k)]
+        return (c[..., 0] * t[0] + c[..., 1] * t[1]) + c[..., 2] * t[2]
+
+    def desc(self, slots, energy_cut=None):
+        """pt_table_desc for pt_upload_table (pointers stay valid while self lives)."""
+        L = len(self.proc)
+        self._procs_c = (ProcDesc * max(L, 1))(*[p.desc(slots) for p in self.proc])
+        d = TableDesc()
+        d.nprocs = L
+        d.order = self.rate.shape[2]
+        d.nintervals = self.b.k
+        d.nswarm = self.ratebound.shape[0]
+        d.xmax = self.b.xmax
+        d.energy_cut = self.energy_cut if energy_cut is None else energy_cut
+        d.rate = self.rate.ctypes.data_as(C.POINTER(C.c_double))
+        d.ratebound = self.ratebound.ctypes.data_as(C.POINTER(C.c_double))
+        d.procs = self._procs_c
+        return d
+
+    def with_ratebounds(self, rbs):
+        """A copy with per-swarm rate bounds [nswarm, k+1, 3]."""
+        return ChebyshevCollisionTable(self.proc, self.b, self.rate, rbs, self.species,
+                                       self.energy_cut, self.sort_keys)
+
+
+def compratebound(species, a, b, Fdt, safety):
+    """src/collision_table.jl:152-167; a is [k+1, L, n] (unsorted)."""
+    L = a.shape[1]
+
+    def f(eng):
+        if L == 0:
+            return np.zeros_like(eng)
+        df = chebdiff(eng, b, a[:, 0, :])
+        fv = chebeval(eng, b, a[:, 0, :])
+        for j in range(1, L):
+            df = df + chebdiff(eng, b, a[:, j, :])
+            fv = fv + chebeval(eng, b, a[:, j, :])
+        g = np.zeros_like(eng) if species == PT_PHOTON else 1.0 + eng / co.electron_mc2
+        return fv + g * g * g * speed(species, eng) * Fdt * np.abs(df) * safety
+
+    return chebfit(f, b, a.shape[2])
+
+
+def _sort_key_order(keys):
+    """sortperm(rev=true) with Julia's isless: NaN first, then descending; stable."""
+    idx = list(range(len(keys)))
+    nan = [i for i in idx if math.isnan(keys[i])]
+    rest = sorted([i for i in idx if not math.isnan(keys[i])], key=lambda i: -keys[i])
+    return nan + rest
+
+
+def collision_table_from_processes(processes, species, Fdt, safety=1.1, nintervals=32, order=3,
+                                   max_energy=DEFAULT_MAX_ENERGY, energy_cut=0.0):
+    """Build a ChebyshevCollisionTable from (coef, process) pairs, src/collision_table.jl:115-149."""
+    b = BinaryIntervals(nintervals, max_energy)
+    L = len(processes)
+    a = np.zeros((nintervals + 1, L, order))
+    for j, (coef, proc) in enumerate(processes):
+        a[:, j, :] = chebfit(lambda e, p=proc, cf=coef: cf * speed(species, e) * p.totalcs(e), b, order)
+    rb = compratebound(species, a, b, Fdt, safety)
+    # LogLinRange(log(1e-2 eV), log(0.9999 max_energy), 100_000), src/util.jl:60-77
+    L1, L2 = math.log(1e-2 * co.eV), math.log(0.9999 * max_energy)
+    t = np.arange(100000, dtype=np.float64) / 99999.0
+    eng = np.exp((1.0 - t) * L1 + t * L2) - math.exp(L1)
+    v1 = speed(species, eng)
+    keys = []
+    for coef, proc in processes:
+        with np.errstate(all="ignore"):
+            vals = proc.totalcs(eng) * v1
+        keys.append(coef * (np.nan if np.isnan(vals).any() else float(np.max(vals))))
+    perm = _sort_key_order(keys)
+    procs = [processes[p][1] for p in perm]
+    return ChebyshevCollisionTable(procs, b, a[:, perm, :], rb, species, energy_cut,
+                                   [keys[p] for p in perm])

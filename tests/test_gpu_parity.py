@@ -1,0 +1,224 @@
+"""GPU parity of the HIP kernels against the CPU oracle, through the C ABI.
+
+Bars: random streams, interval indices, selected processes and rate bounds bit-exact;
+one push step bit-exact (required <= 1e-12 relative); every sampler with injected
+uniforms consumes the same number of uniforms, produces the same outcome kind, species
+and uids, and states within 1e-12 relative (transcendentals come from different libms).
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle as O
+import particulator_hip as P
+from particulator_hip import co
+from particulator_hip._lib import SOA, check, lib
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def tabs():
+    return H.tables()
+
+
+@pytest.fixture(scope="module")
+def dev(tabs):
+    te, tg, tp = tabs
+    say("create device world")
+    mp = H.make_device(te, tg, tp, seed=12345)
+    yield mp
+    mp.close()
+
+
+def test_philox_matches_oracle(dev):
+    rng = np.random.default_rng(1)
+    n = 4096
+    ctr = rng.integers(0, 2 ** 32, size=(n, 4), dtype=np.uint64).astype(np.uint32)
+    ctr[0] = 0
+    ctr[1] = 0xFFFFFFFF
+    key = np.array([0xA4093822, 0x299F31D0], dtype=np.uint32)
+    out = np.zeros((n, 4), dtype=np.uint32)
+    say("pt_debug_philox n=%d" % n)
+    check(dev.ctx, lib().pt_debug_philox(dev.ctx, ctr.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         key.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         out.ctypes.data_as(C.POINTER(C.c_uint32)), n))
+    for i in range(0, n, 97):
+        assert list(out[i]) == O.philox([int(v) for v in ctr[i]], [int(v) for v in key])
+
+
+def test_uniform_stream_matches_oracle(dev):
+    n = 257
+    for uid, step in ((0, 0), (12345678901, 7), (2 ** 63 + 5, 0xFFFFFFFF)):
+        out = np.zeros(n)
+        say("pt_debug_uniforms uid=%d step=%d" % (uid, step))
+        check(dev.ctx, lib().pt_debug_uniforms(dev.ctx, uid, step,
+                                               out.ctypes.data_as(C.POINTER(C.c_double)), n))
+        g = O.Uniforms(seed=12345, uid=uid, step=step)
+        ref = np.array([g() for _ in range(n)])
+        assert np.array_equal(out, ref)
+        assert (out > 0).all() and (out < 1).all()
+
+
+def _energies(rng, n, lo=1e4, hi=2.6e8):
+    E = np.exp(rng.uniform(np.log(lo), np.log(hi), n)) * co.eV
+    # interval edges 2^m keV and their neighbours
+    edges = np.array([1e3 * 2.0 ** m for m in range(3, 18)]) * co.eV
+    E[: edges.size] = edges
+    E[edges.size: 2 * edges.size] = np.nextafter(edges, 0)
+    E[2 * edges.size: 3 * edges.size] = np.nextafter(edges, np.inf)
+    return E
+
+
+@pytest.mark.parametrize("sp", [0, 1, 2])
+def test_select_bitexact(dev, tabs, sp):
+    tb = tabs[sp]
+    ot = H.oracle_table(tb)
+    rng = np.random.default_rng(10 + sp)
+    n = 20000
+    E = _energies(rng, n)
+    u = rng.random(n)
+    iv = np.zeros(n, dtype=np.int32)
+    pr = np.zeros(n, dtype=np.int32)
+    rb = np.zeros(n)
+    say("pt_debug_select species=%d n=%d" % (sp, n))
+    check(dev.ctx, lib().pt_debug_select(dev.ctx, sp, n, E.ctypes.data_as(C.POINTER(C.c_double)),
+                                         u.ctypes.data_as(C.POINTER(C.c_double)), None,
+                                         iv.ctypes.data_as(C.POINTER(C.c_int32)),
+                                         pr.ctypes.data_as(C.POINTER(C.c_int32)),
+                                         rb.ctypes.data_as(C.POINTER(C.c_double))))
+    for i in range(n):
+        ii, t = O.precheb(E[i])
+        assert iv[i] == ii
+        # setr! (src/collisions.jl:63-74): r = 0 below the population's energy cut
+        r = 0.0 if E[i] < tb.energy_cut else O.lib().or_ratebound(C.byref(ot.s), 0, ii, O.dptr(t))
+        assert rb[i] == r
+        j, _ = ot.select(E[i], u[i], r)
+        assert pr[i] == j, (i, E[i], u[i])
+    # the selection spans every process of the table and the null collision
+    assert len(set(pr.tolist())) >= min(len(tb.proc), 6)
+
+
+@pytest.mark.parametrize("sp", [0, 1, 2])
+def test_push_bitexact(dev, sp):
+    rng = np.random.default_rng(20 + sp)
+    n = 4096
+    E = np.exp(rng.uniform(np.log(1e4), np.log(1e8), n)) * co.eV
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    pn = np.array([O.lib().or_momentum_norm_from_kin(sp, e) for e in E])
+    soa = np.zeros((10, n))
+    soa[0:3] = rng.normal(size=(3, n))
+    soa[3:6] = (d * pn[:, None]).T
+    soa[7] = rng.uniform(0, 1e-9, n)
+    dt = rng.uniform(0, 1e-12, n)
+    ref = soa.copy()
+    ptrs = SOA(*[soa[f].ctypes.data_as(C.POINTER(C.c_double)) for f in range(10)])
+    say("pt_debug_push species=%d n=%d" % (sp, n))
+    check(dev.ctx, lib().pt_debug_push(dev.ctx, sp, n, ptrs, dt.ctypes.data_as(C.POINTER(C.c_double))))
+    f = O.field_homogeneous((0.0, 0.0, -1e6))
+    exact = 0
+    for i in range(n):
+        st = O.State()
+        st.x[:] = list(ref[0:3, i])
+        st.p[:] = list(ref[3:6, i])
+        st.t = ref[7, i]
+        O.lib().or_push(sp, C.byref(f), C.byref(st), dt[i])
+        got = np.concatenate([soa[0:3, i], soa[3:6, i], [soa[7, i]]])
+        want = np.array(list(st.x) + list(st.p) + [st.t])
+        scale = np.array([np.abs(want[:3]).max()] * 3 + [np.abs(want[3:6]).max()] * 3 + [abs(want[6])])
+        assert np.all(np.abs(got - want) <= 1e-12 * scale), i
+        exact += np.array_equal(got, want)
+    say("push bit-exact: %d / %d" % (exact, n))
+    assert exact == n
+
+
+def _collide_inputs(sp, proc, rng, n):
+    """states with energies in the process's domain"""
+    lo = {0: 2e4, 1: 2e4, 2: 2e4}[sp]
+    if proc.kind == P.physics.PROC_BETHE_HEITLER:
+        lo = 1.1e6
+    if proc.kind in (P.physics.PROC_MOLLER,):
+        lo = 2.5e4
+    E = np.exp(rng.uniform(np.log(lo), np.log(2e8), n)) * co.eV
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    pn = np.array([O.lib().or_momentum_norm_from_kin(sp, e) for e in E])
+    soa = np.zeros((10, n))
+    soa[0:3] = rng.normal(size=(3, n))
+    soa[3:6] = (d * pn[:, None]).T
+    soa[6] = rng.uniform(0.5, 2.0, n)
+    soa[7] = rng.uniform(0, 1e-9, n)
+    uid = rng.integers(0, 2 ** 62, n, dtype=np.int64).astype(np.uint64)
+    return soa, uid
+
+
+@pytest.mark.parametrize("sp", [0, 1, 2])
+def test_collide_injected(dev, tabs, sp):
+    tb = tabs[sp]
+    stride = 96
+    n = 512
+    for j, proc in enumerate(tb.proc):
+        rng = np.random.default_rng(1000 * sp + j)
+        soa, uid = _collide_inputs(sp, proc, rng, n)
+        u = rng.random((n, stride))
+        kind = np.zeros(n, dtype=np.int32)
+        draws = np.zeros(n, dtype=np.int32)
+        ost = np.zeros((n, 3, 10))
+        osp = np.zeros((n, 3), dtype=np.int32)
+        ouid = np.zeros((n, 3), dtype=np.uint64)
+        ptrs = SOA(*[soa[f].ctypes.data_as(C.POINTER(C.c_double)) for f in range(10)])
+        say("pt_debug_collide species=%d proc=%d (%s)" % (sp, j, type(proc).__name__))
+        check(dev.ctx, lib().pt_debug_collide(
+            dev.ctx, sp, j, n, ptrs, uid.ctypes.data_as(C.POINTER(C.c_uint64)),
+            u.ctypes.data_as(C.POINTER(C.c_double)), stride,
+            kind.ctypes.data_as(C.POINTER(C.c_int32)), draws.ctypes.data_as(C.POINTER(C.c_int32)),
+            ost.ctypes.data_as(C.POINTER(C.c_double)), osp.ctypes.data_as(C.POINTER(C.c_int32)),
+            ouid.ctypes.data_as(C.POINTER(C.c_uint64))))
+        op = H.oracle_proc(proc)
+        step = dev.step
+        worst = 0.0
+        for i in range(n):
+            st = O.State()
+            st.x[:] = list(soa[0:3, i])
+            st.p[:] = list(soa[3:6, i])
+            st.w, st.t, st.uid = soa[6, i], soa[7, i], int(uid[i])
+            st.active = 1
+            g = O.Uniforms(inject=u[i])
+            out = O.Outcome()
+            eng = O.lib().or_kinenergy(sp, st.p)
+            err = O.lib().or_collide(C.byref(op), sp, C.byref(st), eng, 12345, step, g.ref, C.byref(out))
+            if g.rng.exhausted:
+                assert kind[i] == -2
+                continue
+            assert draws[i] == g.draws, (j, i)
+            if err:
+                assert kind[i] < 0
+                continue
+            assert kind[i] == out.kind, (j, i)
+            # primary
+            if out.kind in (1, 2):
+                want = np.array(list(out.st[0].p))
+                got = ost[i, 0, 3:6]
+                e = np.abs(got - want).max() / max(np.abs(want).max(), 1e-300)
+                worst = max(worst, e)
+                assert e <= 1e-12, (j, i, got, want)
+                assert abs(ost[i, 0, 8] - out.st[0].s) <= 1e-12 * abs(out.st[0].s)
+            for q in range(out.nnew):
+                want = np.array(list(out.st[1 + q].p))
+                got = ost[i, 1 + q, 3:6]
+                e = np.abs(got - want).max() / max(np.abs(want).max(), 1e-300)
+                worst = max(worst, e)
+                assert e <= 1e-12, (j, i, q, got, want)
+                assert osp[i, 1 + q] == out.species[1 + q]
+                assert int(ouid[i, 1 + q]) == out.st[1 + q].uid
+                assert abs(ost[i, 1 + q, 8] - out.st[1 + q].s) <= 1e-12 * abs(out.st[1 + q].s)
+        say("  worst relative difference %.3g" % worst)
