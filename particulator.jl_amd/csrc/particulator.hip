@@ -73,48 +73,22 @@ struct Params {
     int lds_doubles;
 };
 
-// ------------------------------------------------------------------ LDS tables
-struct Tabs {
-    const double* rate[PT_NSPECIES];
-    const double* rb[PT_NSPECIES];
+// ------------------------------------------------------------------ LDS state
+// Per-species pointers, table metadata and process records are staged in LDS once per
+// workgroup; the Chebyshev rate tables (and the rate bounds of a single swarm) follow in
+// dynamic LDS.
+struct SpLite {
+    double* f[PT_NFIELDS];
+    uint8_t* active;
+    uint64_t* uid;
+    int32_t* swarm;
+    ull* n;
+    ull cap;
+    const double* rate;  // LDS
+    const double* rb;    // LDS (one swarm) or global
+    double cut, xmax;
+    int tracked, valid, nprocs, k, nswarm, pad;
 };
-
-__device__ void load_tables(const Params& P, double* lds, Tabs& T) {
-    for (int s = 0; s < PT_NSPECIES; s++) {
-        const DevTable& tb = P.tab[s];
-        if (!tb.valid) {
-            T.rate[s] = nullptr;
-            T.rb[s] = nullptr;
-            continue;
-        }
-        int nr = (tb.k + 1) * tb.nprocs * 3;
-        double* dst = lds + P.lds_rate_off[s];
-        for (int q = threadIdx.x; q < nr; q += blockDim.x) dst[q] = tb.rate[q];
-        T.rate[s] = dst;
-        if (P.rb_in_lds) {
-            int nb = (tb.k + 1) * 3;
-            double* d2 = lds + P.lds_rb_off[s];
-            for (int q = threadIdx.x; q < nb; q += blockDim.x) d2[q] = tb.ratebound[q];
-            T.rb[s] = d2;
-        } else {
-            T.rb[s] = tb.ratebound;
-        }
-    }
-    __syncthreads();
-}
-
-// setr!, src/collisions.jl:63-74
-__device__ __forceinline__ double setr(const Params& P, const Tabs& T, int sp, double E, int swarm,
-                                       bool& oob) {
-    const DevTable& tb = P.tab[sp];
-    if (E < tb.cut) return 0.0;
-    double t1, t2;
-    bool o;
-    int i = precheb(E, tb.k, tb.xmax, t1, t2, o);
-    oob |= o;
-    const double* rb = T.rb[sp] + (size_t)(tb.nswarm > 1 ? swarm : 0) * (tb.k + 1) * 3 + i * 3;
-    return cheb3(rb, t1, t2);
-}
 
 // block-local counters, flushed once per block
 struct BlockCounters {
@@ -123,10 +97,59 @@ struct BlockCounters {
     ull work[NWORK];
 };
 
-__device__ void zero_counters(BlockCounters* bc) {
-    ull* p = (ull*)bc;
-    for (int q = threadIdx.x; q < (int)(sizeof(BlockCounters) / sizeof(ull)); q += blockDim.x) p[q] = 0;
+struct Shared {
+    SpLite sp[PT_NSPECIES];
+    DevProc procs[PT_NSPECIES][MAX_PROCS];
+    BlockCounters bc;
+};
+
+__device__ __forceinline__ void stage(const Params& P, double* lds, Shared& sh) {
+    for (int q = threadIdx.x; q < (int)(sizeof(BlockCounters) / sizeof(ull)); q += blockDim.x)
+        ((ull*)&sh.bc)[q] = 0;
+    for (int s = 0; s < PT_NSPECIES; s++) {
+        const DevTable& tb = P.tab[s];
+        const SpeciesDev& S = P.sp[s];
+        if (threadIdx.x == 0) {
+            SpLite& L = sh.sp[s];
+            for (int f = 0; f < PT_NFIELDS; f++) L.f[f] = S.f[f];
+            L.active = S.active;
+            L.uid = S.uid;
+            L.swarm = S.swarm;
+            L.n = S.n;
+            L.cap = S.cap;
+            L.tracked = S.tracked;
+            L.valid = tb.valid;
+            L.nprocs = tb.nprocs;
+            L.k = tb.k;
+            L.nswarm = tb.nswarm;
+            L.cut = tb.cut;
+            L.xmax = tb.xmax;
+            L.rate = lds + P.lds_rate_off[s];
+            L.rb = P.rb_in_lds ? lds + P.lds_rb_off[s] : tb.ratebound;
+        }
+        if (!tb.valid) continue;
+        for (int q = threadIdx.x; q < tb.nprocs; q += blockDim.x) sh.procs[s][q] = tb.procs[q];
+        int nr = (tb.k + 1) * tb.nprocs * 3;
+        double* dst = lds + P.lds_rate_off[s];
+        for (int q = threadIdx.x; q < nr; q += blockDim.x) dst[q] = tb.rate[q];
+        if (P.rb_in_lds) {
+            int nb = (tb.k + 1) * 3;
+            double* d2 = lds + P.lds_rb_off[s];
+            for (int q = threadIdx.x; q < nb; q += blockDim.x) d2[q] = tb.ratebound[q];
+        }
+    }
     __syncthreads();
+}
+
+// setr!, src/collisions.jl:63-74
+__device__ __forceinline__ double setr(const SpLite& L, double E, int swarm, bool& oob) {
+    if (E < L.cut) return 0.0;
+    double t1, t2;
+    bool o;
+    int i = precheb(E, L.k, L.xmax, t1, t2, o);
+    oob |= o;
+    const double* rb = L.rb + (size_t)(L.nswarm > 1 ? swarm : 0) * (L.k + 1) * 3 + i * 3;
+    return cheb3(rb, t1, t2);
 }
 
 __device__ void flush_counters(const Params& P, BlockCounters* bc) {
@@ -141,14 +164,13 @@ __device__ void flush_counters(const Params& P, BlockCounters* bc) {
 
 // ------------------------------------------------------------------ append
 // add_particle!, src/population.jl:103-113 (+ birth tallies of untracked species)
-__device__ void append(const Params& P, const Tabs& T, BlockCounters* bc, int sp2, const V3& x,
-                       const V3& p, double w, double t, double s, uint64_t uid, int swarm,
-                       bool main_pass) {
-    const DevTable& tb = P.tab[sp2];
+__device__ __forceinline__ void append(const Params& P, Shared& sh, int sp2, const V3& x,
+                                       const V3& p, double w, double t, double s, uint64_t uid,
+                                       int swarm, bool main_pass) {
+    const SpLite& L = sh.sp[sp2];
     double E = kinenergy(sp2, p);
-    if (E <= tb.cut) return;
-    const SpeciesDev& S = P.sp[sp2];
-    if (!S.tracked) {
+    if (E <= L.cut) return;
+    if (!L.tracked) {
         if (P.tally) {
             double* h = P.tally + (size_t)sp2 * (P.tally_bins + 2);
             atomicAdd(&h[P.tally_bins], 1.0);
@@ -161,140 +183,183 @@ __device__ void append(const Params& P, const Tabs& T, BlockCounters* bc, int sp
         }
         return;
     }
-    if (!tb.valid) {
-        atomicAdd(&bc->flags[FLAG_CAPACITY], 1ull);
+    if (!L.valid) {
+        atomicAdd(&sh.bc.flags[FLAG_CAPACITY], 1ull);
         return;
     }
-    ull slot = atomicAdd(S.n, 1ull);
-    if (slot >= S.cap) {
-        atomicAdd(&bc->flags[FLAG_CAPACITY], 1ull);
+    ull slot = atomicAdd(L.n, 1ull);
+    if (slot >= L.cap) {
+        atomicAdd(&sh.bc.flags[FLAG_CAPACITY], 1ull);
         return;
     }
     bool oob = false;
-    double r = setr(P, T, sp2, E, swarm, oob);
-    if (oob) atomicAdd(&bc->flags[FLAG_RANGE], 1ull);
-    S.f[PT_X][slot] = x.x;
-    S.f[PT_Y][slot] = x.y;
-    S.f[PT_Z][slot] = x.z;
-    S.f[PT_PX][slot] = p.x;
-    S.f[PT_PY][slot] = p.y;
-    S.f[PT_PZ][slot] = p.z;
-    S.f[PT_W][slot] = w;
-    S.f[PT_T][slot] = t;
-    S.f[PT_S][slot] = s;
-    S.f[PT_R][slot] = r;
-    S.uid[slot] = uid;
-    S.swarm[slot] = swarm;
-    S.active[slot] = 1;
-    atomicAdd(&bc->work[W_SECONDARIES], 1ull);
-    if (main_pass) atomicAdd(&bc->work[W_MAIN_SEC], 1ull);
+    double r = setr(L, E, swarm, oob);
+    if (oob) atomicAdd(&sh.bc.flags[FLAG_RANGE], 1ull);
+    L.f[PT_X][slot] = x.x;
+    L.f[PT_Y][slot] = x.y;
+    L.f[PT_Z][slot] = x.z;
+    L.f[PT_PX][slot] = p.x;
+    L.f[PT_PY][slot] = p.y;
+    L.f[PT_PZ][slot] = p.z;
+    L.f[PT_W][slot] = w;
+    L.f[PT_T][slot] = t;
+    L.f[PT_S][slot] = s;
+    L.f[PT_R][slot] = r;
+    L.uid[slot] = uid;
+    L.swarm[slot] = swarm;
+    L.active[slot] = 1;
+    atomicAdd(&sh.bc.work[W_SECONDARIES], 1ull);
+    if (main_pass) atomicAdd(&sh.bc.work[W_MAIN_SEC], 1ull);
 }
 
-// ------------------------------------------------------------------ one particle
-// The body of advance1!'s @batch loop (src/mixed_population.jl:61-87) with
-// do_one_collision! (src/collisions.jl:142-199) and apply! (:83-133) inlined; r is set
-// first as advance_init! does, and droplow!'s cut is applied at the end.
-__device__ void advance_one(const Params& P, const Tabs& T, BlockCounters* bc, int sp, ull idx,
-                            double tfinal, uint32_t step, bool main_pass) {
-    const SpeciesDev& S = P.sp[sp];
-    if (!S.active[idx]) return;
-    double t = S.f[PT_T][idx];
-    double trem = tfinal - t;
-    if (!(trem > EPS64)) return;
-    V3 x{S.f[PT_X][idx], S.f[PT_Y][idx], S.f[PT_Z][idx]};
-    V3 p{S.f[PT_PX][idx], S.f[PT_PY][idx], S.f[PT_PZ][idx]};
-    double w = S.f[PT_W][idx];
-    double s = S.f[PT_S][idx];
-    uint64_t uid = S.uid[idx];
-    int swarm = S.swarm[idx];
-    if (swarm < 0 || swarm >= P.nswarm) swarm = 0;
-    const DevTable& tb = P.tab[sp];
-    const Field& F = P.fields[swarm];
-    bool oob = false;
-    double E = kinenergy(sp, p);
-    double r = setr(P, T, sp, E, swarm, oob);
+// ------------------------------------------------------------------ particle loop
+// The body of advance1!'s @batch loop (src/mixed_population.jl:61-87), with
+// do_one_collision! (src/collisions.jl:142-199) and apply! (:83-133) inlined, run as an
+// event loop: each iteration advances a lane's particle by one push (and at most one
+// collision); a lane whose particle has reached tfinal loads its next particle at once, so
+// lanes of a wave stay busy whatever their particles' collision counts.  r is recomputed
+// when a particle is loaded (advance_init!) and droplow!'s cut is applied when it is done.
+// Work items q in [start, total) with stride `stride` index the concatenation of the three
+// species' ranges [rb[s], re[s]).
+__device__ __forceinline__ void process_range(const Params& P, Shared& sh, const ull* rb, const ull* re, ull start,
+                              ull stride, double tfinal, uint32_t step, bool main_pass) {
+    const ull n0 = re[0] - rb[0], n1 = re[1] - rb[1], n2 = re[2] - rb[2];
+    const ull total = n0 + n1 + n2;
+    ull q = start;
+    bool have = false;
+    int sp = 0, swarm = 0;
+    ull idx = 0;
+    V3 x{0, 0, 0}, p{0, 0, 0};
+    double w = 0, t = 0, s = 0, r = 0, trem = 0;
+    uint64_t uid = 0;
     PhiloxStream g;
-    g.init(P.seed, uid, step);
-    bool active = true;
-    ull nev = 0;
-    uint32_t iters = 0;
-    while (trem > EPS64 && active) {
-        if (++iters > ITER_LIMIT || g.exhausted()) {
-            atomicAdd(&bc->flags[FLAG_SAMPLER], 1ull);
-            break;
-        }
-        double tnext = s / r;
-        double dt;
-        bool coll;
-        if (trem > tnext) {
-            dt = tnext;
-            coll = true;
-        } else {
-            dt = trem;
-            coll = false;
-            s -= dt * r;
-        }
-        push(sp, F, x, p, t, dt);
-        if (coll && r != 0.0) {
-            E = kinenergy(sp, p);
-            if (E >= tb.cut) {
-                nev++;
-                double t1, t2;
-                bool o;
-                int iv = precheb(E, tb.k, tb.xmax, t1, t2, o);
-                oob |= o;
-                double xi = g.next() * r;
-                const double* rt = T.rate[sp] + (size_t)iv * tb.nprocs * 3;
-                int sel = -1;
-                for (int j = 0; j < tb.nprocs; j++) {
-                    double nu = cheb3(rt + 3 * j, t1, t2);
-                    if (nu > xi) {
-                        sel = j;
-                        break;
-                    }
-                    xi -= nu;
-                }
-                if (sel < 0) {
-                    if (!(xi >= 0.0)) atomicAdd(&bc->flags[FLAG_RATEBOUND], 1ull);
-                    atomicAdd(&bc->counts[sp * NCOUNT], 1ull);
-                    r = setr(P, T, sp, E, swarm, oob);
-                    s = nextcoll(g);
+    g.init(0, 0, 0);
+    uint32_t iters = 0, nev = 0;
+    bool active = false, oob = false;
+    for (;;) {
+        if (!have) {
+            while (q < total) {
+                ull qq = q;
+                q += stride;
+                if (qq < n0) {
+                    sp = 0;
+                    idx = rb[0] + qq;
+                } else if (qq < n0 + n1) {
+                    sp = 1;
+                    idx = rb[1] + (qq - n0);
                 } else {
-                    atomicAdd(&bc->counts[sp * NCOUNT + 1 + sel], 1ull);
-                    Outcome oc;
-                    collide(tb.procs[sel], sp, p, E, P.sb, P.pe, g, oc);
-                    if (oc.err) atomicAdd(&bc->flags[FLAG_SAMPLER], 1ull);
-                    uint32_t dend = g.draw;
-                    if (oc.kind == OUT_STATECHANGE || oc.kind == OUT_NEW) {
-                        p = oc.p0;
-                        s = oc.s0;
-                        E = kinenergy(sp, p);
-                        r = setr(P, T, sp, E, swarm, oob);
-                    } else {
-                        active = false;
+                    sp = 2;
+                    idx = rb[2] + (qq - n0 - n1);
+                }
+                const SpLite& L = sh.sp[sp];
+                if (!L.active[idx]) continue;
+                t = L.f[PT_T][idx];
+                trem = tfinal - t;
+                if (!(trem > EPS64)) continue;
+                x = V3{L.f[PT_X][idx], L.f[PT_Y][idx], L.f[PT_Z][idx]};
+                p = V3{L.f[PT_PX][idx], L.f[PT_PY][idx], L.f[PT_PZ][idx]};
+                w = L.f[PT_W][idx];
+                s = L.f[PT_S][idx];
+                uid = L.uid[idx];
+                swarm = L.swarm[idx];
+                if (swarm < 0 || swarm >= P.nswarm) swarm = 0;
+                oob = false;
+                r = setr(L, kinenergy(sp, p), swarm, oob);
+                g.init(P.seed, uid, step);
+                iters = 0;
+                nev = 0;
+                active = true;
+                have = true;
+                break;
+            }
+            if (!have) break;
+        }
+        const SpLite& L = sh.sp[sp];
+        if (++iters > ITER_LIMIT || g.exhausted()) {
+            atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
+            trem = 0.0;
+        } else {
+            double tnext = s / r;
+            double dt;
+            bool coll;
+            if (trem > tnext) {
+                dt = tnext;
+                coll = true;
+            } else {
+                dt = trem;
+                coll = false;
+                s -= dt * r;
+            }
+            push(sp, P.fields[swarm], x, p, t, dt);
+            if (coll && r != 0.0) {
+                double E = kinenergy(sp, p);
+                if (E >= L.cut) {
+                    nev++;
+                    double t1, t2;
+                    bool o;
+                    int iv = precheb(E, L.k, L.xmax, t1, t2, o);
+                    oob |= o;
+                    double xi = g.next() * r;
+                    const double* rt = L.rate + (size_t)iv * L.nprocs * 3;
+                    int sel = -1;
+                    for (int j = 0; j < L.nprocs; j++) {
+                        double nu = cheb3(rt + 3 * j, t1, t2);
+                        if (nu > xi) {
+                            sel = j;
+                            break;
+                        }
+                        xi -= nu;
                     }
-                    for (int q = 0; q < oc.nnew; q++)
-                        append(P, T, bc, oc.q[q].sp, x, oc.q[q].p, w, t, oc.q[q].s,
-                               child_uid(P.seed, uid, step, dend, q), swarm, main_pass);
+                    if (sel < 0) {
+                        if (!(xi >= 0.0)) atomicAdd(&sh.bc.flags[FLAG_RATEBOUND], 1ull);
+                        atomicAdd(&sh.bc.counts[sp * NCOUNT], 1ull);
+                        r = setr(L, E, swarm, oob);
+                        s = nextcoll(g);
+                    } else {
+                        atomicAdd(&sh.bc.counts[sp * NCOUNT + 1 + sel], 1ull);
+                        double pn = norm3(p);
+                        Ev e;
+                        sample_event(sh.procs[sp][sel], sp, p, pn, E, P.sb, P.pe, g, e);
+                        if (e.err) atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
+                        V3 m0, m1;
+                        double s0, s1;
+                        finish_event(p, pn, e, g, m0, m1, s0, s1);
+                        uint32_t dend = g.draw;
+                        if (e.keep) {
+                            p = m0;
+                            s = s0;
+                            r = setr(L, kinenergy(sp, p), swarm, oob);
+                        } else {
+                            active = false;
+                            append(P, sh, e.sp0, x, m0, w, t, s0, child_uid(P.seed, uid, step, dend, 0),
+                                   swarm, main_pass);
+                        }
+                        if (e.ne == 2)
+                            append(P, sh, e.sp1, x, m1, w, t, s1,
+                                   child_uid(P.seed, uid, step, dend, e.keep ? 0 : 1), swarm, main_pass);
+                    }
                 }
             }
+            trem -= dt;
         }
-        trem -= dt;
+        if (!(trem > EPS64) || !active) {
+            if (active && kinenergy(sp, p) < L.cut) active = false;  // droplow!
+            L.f[PT_X][idx] = x.x;
+            L.f[PT_Y][idx] = x.y;
+            L.f[PT_Z][idx] = x.z;
+            L.f[PT_PX][idx] = p.x;
+            L.f[PT_PY][idx] = p.y;
+            L.f[PT_PZ][idx] = p.z;
+            L.f[PT_T][idx] = t;
+            L.f[PT_S][idx] = s;
+            L.f[PT_R][idx] = r;
+            if (!active) L.active[idx] = 0;
+            if (oob) atomicAdd(&sh.bc.flags[FLAG_RANGE], 1ull);
+            if (nev) atomicAdd(&sh.bc.work[W_EVENTS], (ull)nev);
+            atomicAdd(&sh.bc.work[main_pass ? W_PSTEPS : W_TAIL], 1ull);
+            have = false;
+        }
     }
-    if (active && kinenergy(sp, p) < tb.cut) active = false;  // droplow!
-    S.f[PT_X][idx] = x.x;
-    S.f[PT_Y][idx] = x.y;
-    S.f[PT_Z][idx] = x.z;
-    S.f[PT_PX][idx] = p.x;
-    S.f[PT_PY][idx] = p.y;
-    S.f[PT_PZ][idx] = p.z;
-    S.f[PT_T][idx] = t;
-    S.f[PT_S][idx] = s;
-    S.f[PT_R][idx] = r;
-    if (!active) S.active[idx] = 0;
-    if (oob) atomicAdd(&bc->flags[FLAG_RANGE], 1ull);
-    if (nev) atomicAdd(&bc->work[W_EVENTS], nev);
-    atomicAdd(&bc->work[main_pass ? W_PSTEPS : W_TAIL], 1ull);
 }
 
 // ------------------------------------------------------------------ step kernels
@@ -312,48 +377,32 @@ __global__ void k_snapshot(const Params* __restrict__ Pp) {
 __global__ void __launch_bounds__(256) k_advance(const Params* __restrict__ Pp, double tfinal,
                                                  uint32_t step) {
     extern __shared__ double lds[];
-    __shared__ BlockCounters bc;
+    __shared__ Shared sh;
     const Params& P = *Pp;
-    Tabs T;
-    zero_counters(&bc);
-    load_tables(P, lds, T);
-    ull b0 = P.range[0], e0 = P.range[1], b1 = P.range[2], e1 = P.range[3], b2 = P.range[4],
-        e2 = P.range[5];
-    ull n0 = e0 - b0, n1 = e1 - b1, n2 = e2 - b2;
-    ull total = n0 + n1 + n2;
-    ull stride = (ull)gridDim.x * blockDim.x;
-    for (ull q = (ull)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
-        int sp;
-        ull idx;
-        if (q < n0) {
-            sp = 0;
-            idx = b0 + q;
-        } else if (q < n0 + n1) {
-            sp = 1;
-            idx = b1 + (q - n0);
-        } else {
-            sp = 2;
-            idx = b2 + (q - n0 - n1);
-        }
-        advance_one(P, T, &bc, sp, idx, tfinal, step, true);
+    stage(P, lds, sh);
+    ull rb[PT_NSPECIES], re[PT_NSPECIES];
+    for (int s = 0; s < PT_NSPECIES; s++) {
+        rb[s] = P.range[2 * s];
+        re[s] = P.range[2 * s + 1];
     }
-    flush_counters(P, &bc);
+    process_range(P, sh, rb, re, (ull)blockIdx.x * blockDim.x + threadIdx.x,
+                  (ull)gridDim.x * blockDim.x, tfinal, step, true);
+    flush_counters(P, &sh.bc);
 }
 
 // Secondaries born in this step: one workgroup repeats passes over [begin, n) until a pass
 // appends nothing (the `while n > 0` loop of advance!, src/mixed_population.jl:43-46).
-__global__ void __launch_bounds__(1024) k_tail(const Params* __restrict__ Pp, double tfinal,
-                                               uint32_t step, int maxpasses) {
+__global__ void __launch_bounds__(512) k_tail(const Params* __restrict__ Pp, double tfinal,
+                                              uint32_t step, int maxpasses) {
     extern __shared__ double lds[];
-    __shared__ BlockCounters bc;
+    __shared__ Shared sh;
     __shared__ ull beg[PT_NSPECIES], end[PT_NSPECIES];
     __shared__ int done;
     const Params& P = *Pp;
-    Tabs T;
-    zero_counters(&bc);
-    load_tables(P, lds, T);
+    stage(P, lds, sh);
     if (threadIdx.x == 0)
         for (int s = 0; s < PT_NSPECIES; s++) beg[s] = P.range[2 * s + 1];
+    if (threadIdx.x == 0) done = 0;
     for (int pass = 0; pass < maxpasses; pass++) {
         __threadfence();
         __syncthreads();
@@ -368,49 +417,38 @@ __global__ void __launch_bounds__(1024) k_tail(const Params* __restrict__ Pp, do
         }
         __syncthreads();
         if (done) break;
-        ull n0 = end[0] - beg[0], n1 = end[1] - beg[1], n2 = end[2] - beg[2];
-        ull total = n0 + n1 + n2;
-        for (ull q = threadIdx.x; q < total; q += blockDim.x) {
-            int sp;
-            ull idx;
-            if (q < n0) {
-                sp = 0;
-                idx = beg[0] + q;
-            } else if (q < n0 + n1) {
-                sp = 1;
-                idx = beg[1] + (q - n0);
-            } else {
-                sp = 2;
-                idx = beg[2] + (q - n0 - n1);
-            }
-            advance_one(P, T, &bc, sp, idx, tfinal, step, false);
+        ull rb[PT_NSPECIES], re[PT_NSPECIES];
+        for (int s = 0; s < PT_NSPECIES; s++) {
+            rb[s] = beg[s];
+            re[s] = end[s];
         }
+        process_range(P, sh, rb, re, threadIdx.x, blockDim.x, tfinal, step, false);
         __threadfence();
         __syncthreads();
         if (threadIdx.x == 0)
             for (int s = 0; s < PT_NSPECIES; s++) beg[s] = end[s];
     }
     // the pass limit was reached with work left: report it
-    if (threadIdx.x == 0 && !done) bc.flags[FLAG_TAIL] += 1;
-    flush_counters(P, &bc);
+    if (threadIdx.x == 0 && !done) sh.bc.flags[FLAG_TAIL] += 1;
+    flush_counters(P, &sh.bc);
 }
 
 // init! / advance_init!: r of every active particle (src/mixed_population.jl:20-35)
 __global__ void __launch_bounds__(256) k_setr(const Params* __restrict__ Pp, int sp) {
     extern __shared__ double lds[];
+    __shared__ Shared sh;
     const Params& P = *Pp;
-    Tabs T;
-    load_tables(P, lds, T);
-    const SpeciesDev& S = P.sp[sp];
-    ull n = *S.n;
-    if (n > S.cap) n = S.cap;
+    stage(P, lds, sh);
+    const SpLite& L = sh.sp[sp];
+    ull n = *L.n;
+    if (n > L.cap) n = L.cap;
     for (ull i = (ull)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ull)gridDim.x * blockDim.x) {
-        if (!S.active[i]) continue;
-        V3 p{S.f[PT_PX][i], S.f[PT_PY][i], S.f[PT_PZ][i]};
-        int sw = S.swarm[i];
+        if (!L.active[i]) continue;
+        V3 p{L.f[PT_PX][i], L.f[PT_PY][i], L.f[PT_PZ][i]};
+        int sw = L.swarm[i];
         if (sw < 0 || sw >= P.nswarm) sw = 0;
         bool oob = false;
-        S.f[PT_R][i] = setr(P, T, sp, kinenergy(sp, p), sw, oob);
+        L.f[PT_R][i] = setr(L, kinenergy(sp, p), sw, oob);
         if (oob) atomicAdd(&P.flags[FLAG_RANGE], 1ull);
     }
 }
@@ -687,23 +725,23 @@ __global__ void __launch_bounds__(256) k_dbg_select(const Params* __restrict__ P
                                                     const int32_t* swarm, int32_t* interval,
                                                     int32_t* proc, double* rbound) {
     extern __shared__ double lds[];
+    __shared__ Shared sh;
     const Params& P = *Pp;
-    Tabs T;
-    load_tables(P, lds, T);
+    stage(P, lds, sh);
     ull i = (ull)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const DevTable& tb = P.tab[sp];
+    const SpLite& L = sh.sp[sp];
     int sw = swarm ? swarm[i] : 0;
     if (sw < 0 || sw >= P.nswarm) sw = 0;
     bool oob = false;
-    double r = setr(P, T, sp, E[i], sw, oob);
+    double r = setr(L, E[i], sw, oob);
     double t1, t2;
     bool o;
-    int iv = precheb(E[i], tb.k, tb.xmax, t1, t2, o);
+    int iv = precheb(E[i], L.k, L.xmax, t1, t2, o);
     double xi = u[i] * r;
-    const double* rt = T.rate[sp] + (size_t)iv * tb.nprocs * 3;
+    const double* rt = L.rate + (size_t)iv * L.nprocs * 3;
     int sel = -1;
-    for (int j = 0; j < tb.nprocs; j++) {
+    for (int j = 0; j < L.nprocs; j++) {
         double nu = cheb3(rt + 3 * j, t1, t2);
         if (nu > xi) {
             sel = j;
@@ -735,6 +773,7 @@ __global__ void __launch_bounds__(256) k_dbg_push(const Params* __restrict__ Pp,
     f7[i] = t;
 }
 
+// collide() with injected uniforms through the same sample_event / finish_event as the step
 __global__ void __launch_bounds__(256) k_dbg_collide(const Params* __restrict__ Pp, int sp, int proc,
                                                      ull n, const double* in, const uint64_t* uid,
                                                      const double* u, int stride, uint32_t step,
@@ -753,9 +792,14 @@ __global__ void __launch_bounds__(256) k_dbg_collide(const Params* __restrict__ 
     g.draw = 0;
     g.ran_out = false;
     double E = kinenergy(sp, p);
-    Outcome oc;
-    collide(P.tab[sp].procs[proc], sp, p, E, P.sb, P.pe, g, oc);
-    okind[i] = g.ran_out ? -2 : (oc.err ? -1 - 16 * oc.err : oc.kind);
+    double pn = norm3(p);
+    Ev e;
+    sample_event(P.tab[sp].procs[proc], sp, p, pn, E, P.sb, P.pe, g, e);
+    V3 m0, m1;
+    double s0, s1;
+    finish_event(p, pn, e, g, m0, m1, s0, s1);
+    int kind = e.keep ? (e.ne == 1 ? 1 : 2) : (e.ne == 1 ? 3 : 4);
+    okind[i] = g.ran_out ? -2 : (e.err ? -1 - 16 * e.err : kind);
     odraws[i] = (int32_t)g.draw;
     double* o = ost + i * 3 * PT_NFIELDS;
     for (int q = 0; q < 3 * PT_NFIELDS; q++) o[q] = 0.0;
@@ -763,17 +807,17 @@ __global__ void __launch_bounds__(256) k_dbg_collide(const Params* __restrict__ 
         osp[i * 3 + q] = -1;
         ouid[i * 3 + q] = 0;
     }
-    if (oc.kind == OUT_STATECHANGE || oc.kind == OUT_NEW) {
-        double v[PT_NFIELDS] = {x.x, x.y, x.z, oc.p0.x, oc.p0.y, oc.p0.z, w, t, oc.s0, 0.0};
-        for (int f = 0; f < PT_NFIELDS; f++) o[f] = v[f];
-        osp[i * 3] = sp;
-        ouid[i * 3] = uid[i];
-    }
-    for (int q = 0; q < oc.nnew; q++) {
-        double v[PT_NFIELDS] = {x.x, x.y, x.z, oc.q[q].p.x, oc.q[q].p.y, oc.q[q].p.z, w, t, oc.q[q].s, 0.0};
-        for (int f = 0; f < PT_NFIELDS; f++) o[(1 + q) * PT_NFIELDS + f] = v[f];
-        osp[i * 3 + 1 + q] = oc.q[q].sp;
-        ouid[i * 3 + 1 + q] = child_uid(P.seed, uid[i], step, g.draw, q);
+    // slot 0: the surviving primary; slots 1, 2: new particles in creation order
+    V3 mom[2] = {m0, m1};
+    double ss[2] = {s0, s1};
+    int sps[2] = {e.sp0, e.sp1};
+    int slot = 1;
+    for (int q = 0; q < e.ne; q++) {
+        int d = (q == 0 && e.keep) ? 0 : slot++;
+        double v[PT_NFIELDS] = {x.x, x.y, x.z, mom[q].x, mom[q].y, mom[q].z, w, t, ss[q], 0.0};
+        for (int f = 0; f < PT_NFIELDS; f++) o[d * PT_NFIELDS + f] = v[f];
+        osp[i * 3 + d] = sps[q];
+        ouid[i * 3 + d] = d == 0 ? uid[i] : child_uid(P.seed, uid[i], step, g.draw, d - 1);
     }
 }
 
@@ -1340,7 +1384,7 @@ static int launch_step(pt_ctx* c, double tfinal) {
     hipLaunchKernelGGL(k_advance, dim3(blocks), dim3(256), c->lds_bytes, c->stream, c->dp, tfinal, c->step);
     HIPCHK(c, hipGetLastError());
     if (c->profile) HIPCHK(c, hipEventRecord(e1, c->stream));
-    hipLaunchKernelGGL(k_tail, dim3(1), dim3(1024), c->lds_bytes, c->stream, c->dp, tfinal, c->step, 100000);
+    hipLaunchKernelGGL(k_tail, dim3(1), dim3(512), c->lds_bytes, c->stream, c->dp, tfinal, c->step, 100000);
     HIPCHK(c, hipGetLastError());
     if (c->profile) HIPCHK(c, hipEventRecord(e2, c->stream));
     c->step++;

@@ -271,7 +271,7 @@ __device__ __forceinline__ V3 randsphere(G& g) {
 // ---------------------------------------------------------------- samplers
 // sample_modified_tsai_cos_theta, src/util.jl:143-159
 template <class G>
-__device__ double sample_tsai(double T, G& g) {
+__device__ __forceinline__ double sample_tsai(double T, G& g) {
     double umax = 2.0 * (1.0 + T / (ME * C2));
     const double a1 = 1.6, a2 = 1.6 / 3.0, border = 0.25;
     double u;
@@ -287,7 +287,7 @@ __device__ double sample_tsai(double T, G& g) {
 
 // sample_rel_sr, src/relativistic_coulomb.jl:35-51
 template <class G>
-__device__ double sample_rel_sr(double alpha, double beta, G& g) {
+__device__ __forceinline__ double sample_rel_sr(double alpha, double beta, G& g) {
     double x;
     for (;;) {
         double u = g.next();
@@ -300,7 +300,7 @@ __device__ double sample_rel_sr(double alpha, double beta, G& g) {
 
 // rbeb_sample / _rbeb_sample, src/rbeb.jl:156-196
 template <class G>
-__device__ double rbeb_sample(double T, double B, G& g) {
+__device__ __forceinline__ double rbeb_sample(double T, double B, G& g) {
     double t1 = T / (ME * C2);
     double b1 = B / (ME * C2);
     double bt2 = 1.0 - 1.0 / ((1.0 + t1) * (1.0 + t1));
@@ -329,7 +329,7 @@ __device__ double rbeb_sample(double T, double B, G& g) {
 
 // sample_secondary_energy(::Moller), src/moller.jl:64-87
 template <class G>
-__device__ double moller_sample(double tcut, double eng, G& g) {
+__device__ __forceinline__ double moller_sample(double tcut, double eng, G& g) {
     double e0 = tcut / eng;
     double gm = 1.0 + eng / MC2;
     double pre = 4.0 / (9.0 * (gm * gm) - 10.0 * gm + 5.0);
@@ -359,7 +359,7 @@ __device__ __forceinline__ void bhabha_bs(double y, double& B1, double& B2, doub
 
 // sample_secondary_energy(::Bhaba), src/bhaba.jl:55-81
 template <class G>
-__device__ double bhabha_sample(double tcut, double eng, G& g) {
+__device__ __forceinline__ double bhabha_sample(double tcut, double eng, G& g) {
     double e0 = tcut / eng;
     double gm = 1.0 + eng / MC2;
     double y = 1.0 / (gm + 1.0);
@@ -393,7 +393,7 @@ __device__ __forceinline__ int pcum_first(double x, int ncum) {
 // sample_secondary_energy(::SeltzerBerger), src/seltzer.jl:97-122.  Returns k/T, or a
 // negative value where the reference indexes out of bounds.
 template <class G>
-__device__ double sb_sample(const DevSB& sb, double T, G& g) {
+__device__ __forceinline__ double sb_sample(const DevSB& sb, double T, G& g) {
     double x = g.next();
     double y = log(T);
     int ncum = sb.ncum;
@@ -421,7 +421,7 @@ __device__ double sb_sample(const DevSB& sb, double T, G& g) {
 
 // sample_secondary_energy_and_cos_theta(::Compton), src/compton.jl:119-144
 template <class G>
-__device__ void compton_sample(double eng, G& g, double& E1, double& cost) {
+__device__ __forceinline__ void compton_sample(double eng, G& g, double& E1, double& cost) {
     double e0 = MC2 / (MC2 + 2.0 * eng);
     double a1 = -log(e0);
     double a2 = (1.0 - e0 * e0) / 2.0;
@@ -440,7 +440,7 @@ __device__ void compton_sample(double eng, G& g, double& E1, double& cost) {
 
 // sample_electron_cos_theta(::PhotoElectric), src/photo_electric.jl:78-99
 template <class G>
-__device__ double sauter_sample(double ee, G& g) {
+__device__ __forceinline__ double sauter_sample(double ee, G& g) {
     double gm = 1.0 + ee / MC2;
     double b = sqrt(1.0 - 1.0 / (gm * gm));
     double A = 1.0 / b - 1.0;
@@ -474,7 +474,7 @@ __device__ __forceinline__ double jmax(double a, double b) {
 // fz_lo = 8 log(Z)/3, fz_hi = fz_lo + 8 fc (precomputed by the host).  Returns the
 // reference's (ekin, pkin).
 template <class G>
-__device__ void bh_sample(double cz, double fz_lo, double fz_hi, double eng, G& g,
+__device__ __forceinline__ void bh_sample(double cz, double fz_lo, double fz_hi, double eng, G& g,
                           double& ekin, double& pkin) {
     double e0 = MC2 / eng;
     double eps;
@@ -520,7 +520,7 @@ __device__ void bh_sample(double cz, double fz_lo, double fz_hi, double eng, G& 
 
 // sample_secondary_energy_fraction(::PositronAnihilation), src/anihilation.jl:39-59
 template <class G>
-__device__ double annih_sample(double eng, G& g) {
+__device__ __forceinline__ double annih_sample(double eng, G& g) {
     double gm = 1.0 + eng / MC2;
     double sq = sqrt((gm - 1.0) / (gm + 1.0));
     double emax = (1.0 + sq) / 2.0, emin = (1.0 - sq) / 2.0;
@@ -588,54 +588,49 @@ __device__ __forceinline__ void push(int sp, const Field& f, V3& x, V3& p, doubl
 }
 
 // ---------------------------------------------------------------- collide
-// The outcome of one real collision: the primary's new state (if it survives) and up to
-// two new particles (src/collisions.jl:11-55).
-enum { OUT_NULL = 0, OUT_STATECHANGE = 1, OUT_NEW = 2, OUT_REPLACE = 3, OUT_REPLACE_PAIR = 4 };
+// One real collision, split in two parts so that lanes of a wave that sample different
+// processes still share the expensive arithmetic:
+//   sample_event: the process-specific draws (energy sharing, polar angles, azimuth) in
+//                 the reference's order, producing at most two rotations of the primary's
+//                 momentum, turn A = (cA, phi, nA) and turn B = (cB, sgnB*phi, nB);
+//   finish_event: both rotations (`turn`, src/util.jl:40-57, sharing the primary's
+//                 direction and sincos(phi)) and the constructors' s = nextcoll() draws.
+// Every outgoing particle takes turn A, turn B or p - (turn A), listed in constructor order
+// (which is the order of the s draws).  The primary either survives as entry 0 (`keep`)
+// or is removed (ReplaceParticle*Outcome, src/collisions.jl:112-133).
+enum { SRC_A = 0, SRC_B = 1, SRC_PMA = 2 };
 
-struct NewPart {
-    V3 p;
-    double s;
-    int sp;
-    uint64_t uid;
-};
-
-struct Outcome {
-    int kind;
-    int nnew;
-    V3 p0;      // primary's new momentum (STATECHANGE, NEW)
-    double s0;  // primary's new s
-    NewPart q[2];
+struct Ev {
+    double cA, nA, cB, nB, phi;
+    int nturn, sgnB, keep, ne;
+    int src0, src1, sp0, sp1;
     int err;
 };
 
-// Lehtinen 1999 angles of the two-electron processes (src/rbeb.jl:65-76,
-// src/moller.jl:218-229, src/bhaba.jl:311-322)
-__device__ __forceinline__ void two_body(const V3& p, double E0, double E1, double E2, double phi,
-                                         V3& p1v, V3& p2v) {
-    double p1 = sqrt(E1 * E1 + 2.0 * MC2 * E1) / C;
-    double p2 = sqrt(E2 * E2 + 2.0 * MC2 * E2) / C;
-    double c1 = sqrt(E1 * (E0 + 2.0 * MC2) / (E0 * (E1 + 2.0 * MC2)));
-    double c2 = sqrt(E2 * (E0 + 2.0 * MC2) / (E0 * (E2 + 2.0 * MC2)));
-    p1v = turn(p, c1, phi, p1);
-    p2v = turn(p, c2, -phi, p2);
-}
-
 template <class G>
-__device__ void collide(const DevProc& pr, int sp, const V3& p, double eng, const DevSB* sbs,
-                        const DevPE* pes, G& g, Outcome& o) {
-    o.err = 0;
-    o.nnew = 0;
+__device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3& p, double pnorm,
+                                             double eng, const DevSB* sbs, const DevPE* pes, G& g,
+                                             Ev& e) {
+    e.err = 0;
+    e.nturn = 1;
+    e.sgnB = 1;
+    e.cB = 0.0;
+    e.nB = 0.0;
+    e.src1 = SRC_A;
+    e.sp1 = sp;
     switch (pr.kind) {
     case PT_PROC_ELASTIC: {  // src/relativistic_coulomb.jl:10-23
-        double phi = TWO_PI * g.next();
+        e.phi = TWO_PI * g.next();
         V3 v = velocity(sp, p);
         double beta = norm3(v) / C;
         double a = pr.c[0];
         double al = HBAR * HBAR / (4.0 * dot3(p, p) * (a * a));
-        double cost = sample_rel_sr(al, beta, g);
-        o.p0 = turn(p, cost, phi, norm3(p));
-        o.s0 = nextcoll(g);
-        o.kind = OUT_STATECHANGE;
+        e.cA = sample_rel_sr(al, beta, g);
+        e.nA = pnorm;
+        e.keep = 1;
+        e.ne = 1;
+        e.src0 = SRC_A;
+        e.sp0 = sp;
         break;
     }
     case PT_PROC_RBEB:  // src/rbeb.jl:54-80
@@ -645,55 +640,56 @@ __device__ void collide(const DevProc& pr, int sp, const V3& p, double eng, cons
         if (pr.kind == PT_PROC_RBEB) {
             E2 = rbeb_sample(E0, pr.B, g);
             E1 = E0 - E2 - pr.B;
-            if (!(E2 < E1)) o.err = 1;
+            if (!(E2 < E1)) e.err = 1;
         } else {
-            E2 = pr.kind == PT_PROC_MOLLER ? moller_sample(pr.tcut, eng, g)
-                                           : bhabha_sample(pr.tcut, eng, g);
+            E2 = pr.kind == PT_PROC_MOLLER ? moller_sample(pr.tcut, eng, g) : bhabha_sample(pr.tcut, eng, g);
             E1 = E0 - E2;
         }
-        double phi = TWO_PI * g.next();
-        V3 p1v, p2v;
-        two_body(p, E0, E1, E2, phi, p1v, p2v);
-        o.kind = OUT_NEW;
-        o.p0 = p1v;
-        o.s0 = nextcoll(g);
-        o.q[0].p = p2v;
-        o.q[0].s = nextcoll(g);
-        o.q[0].sp = PT_ELECTRON;
-        o.nnew = 1;
+        e.phi = TWO_PI * g.next();
+        // Lehtinen 1999 angles
+        e.nA = sqrt(E1 * E1 + 2.0 * MC2 * E1) / C;
+        e.nB = sqrt(E2 * E2 + 2.0 * MC2 * E2) / C;
+        e.cA = sqrt(E1 * (E0 + 2.0 * MC2) / (E0 * (E1 + 2.0 * MC2)));
+        e.cB = sqrt(E2 * (E0 + 2.0 * MC2) / (E0 * (E2 + 2.0 * MC2)));
+        e.nturn = 2;
+        e.sgnB = -1;
+        e.keep = 1;
+        e.ne = 2;
+        e.src0 = SRC_A;
+        e.sp0 = sp;
+        e.src1 = SRC_B;
+        e.sp1 = PT_ELECTRON;
         break;
     }
     case PT_PROC_SB: {  // src/seltzer.jl:67-90
         double kf = sb_sample(sbs[pr.slot], eng, g);
-        if (kf < 0.0) o.err = 4;
+        if (kf < 0.0) e.err = 4;
         double k = eng * kf;
-        if (!(k < eng)) o.err = 2;
-        double pnorm = k / C;
-        double cost = sample_tsai(eng, g);
-        double phi = TWO_PI * g.next();
-        V3 pph = turn(p, cost, phi, pnorm);
-        o.kind = OUT_NEW;
-        o.p0 = V3{p.x - pph.x, p.y - pph.y, p.z - pph.z};
-        o.s0 = nextcoll(g);
-        o.q[0].p = pph;
-        o.q[0].s = nextcoll(g);
-        o.q[0].sp = PT_PHOTON;
-        o.nnew = 1;
+        if (!(k < eng)) e.err = 2;
+        e.nA = k / C;
+        e.cA = sample_tsai(eng, g);
+        e.phi = TWO_PI * g.next();
+        e.keep = 1;
+        e.ne = 2;
+        e.src0 = SRC_PMA;
+        e.sp0 = PT_ELECTRON;
+        e.src1 = SRC_A;
+        e.sp1 = PT_PHOTON;
         break;
     }
     case PT_PROC_COMPTON:
     case PT_PROC_KN_COMPTON: {  // src/compton.jl:9-28
         double E1, cost;
         compton_sample(eng, g, E1, cost);
-        double phi = TWO_PI * g.next();
-        V3 pg = turn(p, cost, phi, E1 / C);
-        o.kind = OUT_NEW;
-        o.p0 = pg;
-        o.s0 = nextcoll(g);
-        o.q[0].p = V3{p.x - pg.x, p.y - pg.y, p.z - pg.z};
-        o.q[0].s = nextcoll(g);
-        o.q[0].sp = PT_ELECTRON;
-        o.nnew = 1;
+        e.cA = cost;
+        e.phi = TWO_PI * g.next();
+        e.nA = E1 / C;
+        e.keep = 1;
+        e.ne = 2;
+        e.src0 = SRC_A;
+        e.sp0 = PT_PHOTON;
+        e.src1 = SRC_PMA;
+        e.sp1 = PT_ELECTRON;
         break;
     }
     case PT_PROC_PHOTOELECTRIC: {  // src/photo_electric.jl:38-52, :60-76
@@ -703,60 +699,116 @@ __device__ void collide(const DevProc& pr, int sp, const V3& p, double eng, cons
             b = pe.binding[q];
             if (eng > b) break;
         }
-        if (!(eng > b)) o.err = 3;
+        if (!(eng > b)) e.err = 3;
         double ee = eng - b;
-        double cost = sauter_sample(ee, g);
-        double phi = TWO_PI * g.next();
-        double pn = mom_from_kin(PT_ELECTRON, ee);
-        o.kind = OUT_REPLACE;
-        o.q[0].p = turn(p, cost, phi, pn);
-        o.q[0].s = nextcoll(g);
-        o.q[0].sp = PT_ELECTRON;
-        o.nnew = 1;
+        e.cA = sauter_sample(ee, g);
+        e.phi = TWO_PI * g.next();
+        e.nA = mom_from_kin(PT_ELECTRON, ee);
+        e.keep = 0;
+        e.ne = 1;
+        e.src0 = SRC_A;
+        e.sp0 = PT_ELECTRON;
         break;
     }
     case PT_PROC_BETHE_HEITLER: {  // src/bethe_heitler.jl:5-25 (names swapped as there)
         double pkin, ekin;
         bh_sample(pr.c[0], pr.c[1], pr.c[2], eng, g, pkin, ekin);
-        double phi = TWO_PI * g.next();
-        double cost = sample_tsai(ekin, g);
-        double pn = mom_from_kin(PT_ELECTRON, ekin);
-        V3 pe = turn(p, cost, phi, pn);
-        cost = sample_tsai(pkin, g);
-        pn = mom_from_kin(PT_POSITRON, pkin);
-        V3 pp = turn(p, cost, phi, pn);
-        o.kind = OUT_REPLACE_PAIR;
-        o.q[0].p = pe;
-        o.q[0].s = nextcoll(g);
-        o.q[0].sp = PT_ELECTRON;
-        o.q[1].p = pp;
-        o.q[1].s = nextcoll(g);
-        o.q[1].sp = PT_POSITRON;
-        o.nnew = 2;
+        e.phi = TWO_PI * g.next();
+        e.cA = sample_tsai(ekin, g);
+        e.nA = mom_from_kin(PT_ELECTRON, ekin);
+        e.cB = sample_tsai(pkin, g);
+        e.nB = mom_from_kin(PT_POSITRON, pkin);
+        e.nturn = 2;
+        e.sgnB = 1;
+        e.keep = 0;
+        e.ne = 2;
+        e.src0 = SRC_A;
+        e.sp0 = PT_ELECTRON;
+        e.src1 = SRC_B;
+        e.sp1 = PT_POSITRON;
         break;
     }
     case PT_PROC_ANNIHILATION: {  // src/anihilation.jl:6-23, :61-67
         double eps = annih_sample(eng, g);
         double gm = 1.0 + eng / MC2;
-        double cost = (eps * (gm + 1.0) - 1.0) / (eps * sqrt(gm * gm - 1.0));
-        double phi = TWO_PI * g.next();
+        e.cA = (eps * (gm + 1.0) - 1.0) / (eps * sqrt(gm * gm - 1.0));
+        e.phi = TWO_PI * g.next();
         double etot = eng + 2.0 * MC2;
-        double pan = eps * etot / C;
-        V3 pa = turn(p, cost, phi, pan);
-        o.kind = OUT_REPLACE_PAIR;
-        o.q[0].p = pa;
-        o.q[0].s = nextcoll(g);
-        o.q[0].sp = PT_PHOTON;
-        o.q[1].p = V3{p.x - pa.x, p.y - pa.y, p.z - pa.z};
-        o.q[1].s = nextcoll(g);
-        o.q[1].sp = PT_PHOTON;
-        o.nnew = 2;
+        e.nA = eps * etot / C;
+        e.keep = 0;
+        e.ne = 2;
+        e.src0 = SRC_A;
+        e.sp0 = PT_PHOTON;
+        e.src1 = SRC_PMA;
+        e.sp1 = PT_PHOTON;
         break;
     }
     default:
-        o.kind = OUT_NULL;
+        e.err = 5;
+        e.phi = 0.0;
+        e.cA = 1.0;
+        e.nA = pnorm;
+        e.keep = 1;
+        e.ne = 1;
+        e.src0 = SRC_A;
+        e.sp0 = sp;
         break;
     }
+}
+
+// turn (src/util.jl:40-57) of p by turn A and, if requested, turn B; momenta of the
+// outgoing entries and the constructors' s draws (src/electron.jl:32, src/photon.jl:30)
+template <class G>
+__device__ __forceinline__ void finish_event(const V3& p, double pnorm, const Ev& e, G& g, V3& m0,
+                                             V3& m1, double& s0, double& s1) {
+    double u0 = p.x / pnorm, u1 = p.y / pnorm, u2 = p.z / pnorm;
+    double sp, cp;
+    sincos(e.phi, &sp, &cp);
+    double qs = 1.0 - u2 * u2;
+    double s = sqrt(qs > 0.0 ? qs : 0.0);
+    V3 pa, pb;
+    {
+        double q = 1.0 - e.cA * e.cA;
+        double st = sqrt(q > 0.0 ? q : 0.0);
+        double x, y, z;
+        if (s > 0.0) {
+            double bx = u0 * u2 * cp + (-u1) * sp;
+            double by = u1 * u2 * cp + u0 * sp;
+            x = st * bx / s + u0 * e.cA;
+            y = st * by / s + u1 * e.cA;
+            z = -s * st * cp + u2 * e.cA;
+        } else {
+            x = st * cp;
+            y = st * sp;
+            z = u2 * e.cA;
+        }
+        pa = V3{e.nA * x, e.nA * y, e.nA * z};
+    }
+    if (e.nturn == 2) {
+        double spb = e.sgnB > 0 ? sp : -sp;
+        double q = 1.0 - e.cB * e.cB;
+        double st = sqrt(q > 0.0 ? q : 0.0);
+        double x, y, z;
+        if (s > 0.0) {
+            double bx = u0 * u2 * cp + (-u1) * spb;
+            double by = u1 * u2 * cp + u0 * spb;
+            x = st * bx / s + u0 * e.cB;
+            y = st * by / s + u1 * e.cB;
+            z = -s * st * cp + u2 * e.cB;
+        } else {
+            x = st * cp;
+            y = st * spb;
+            z = u2 * e.cB;
+        }
+        pb = V3{e.nB * x, e.nB * y, e.nB * z};
+    } else {
+        pb = V3{0.0, 0.0, 0.0};
+    }
+    V3 pma{p.x - pa.x, p.y - pa.y, p.z - pa.z};
+    s0 = nextcoll(g);
+    s1 = e.ne == 2 ? nextcoll(g) : 0.0;
+    m0 = e.src0 == SRC_A ? pa : (e.src0 == SRC_B ? pb : pma);
+    m1 = e.src1 == SRC_A ? pa : (e.src1 == SRC_B ? pb : pma);
 }
 
 }  // namespace pt
