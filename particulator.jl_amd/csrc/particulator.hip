@@ -164,12 +164,15 @@ __device__ void flush_counters(const Params& P, BlockCounters* bc) {
 
 // ------------------------------------------------------------------ append
 // add_particle!, src/population.jl:103-113 (+ birth tallies of untracked species)
-__device__ __forceinline__ void append(const Params& P, Shared& sh, int sp2, const V3& x,
-                                       const V3& p, double w, double t, double s, uint64_t uid,
-                                       int swarm, bool main_pass) {
+constexpr ull NO_SLOT = ~0ull;
+constexpr int NCHILD = 2;
+
+__device__ __forceinline__ ull append(const Params& P, Shared& sh, int sp2, const V3& x,
+                                      const V3& p, double w, double t, double s, uint64_t uid,
+                                      int swarm, bool main_pass) {
     const SpLite& L = sh.sp[sp2];
     double E = kinenergy(sp2, p);
-    if (E <= L.cut) return;
+    if (E <= L.cut) return NO_SLOT;
     if (!L.tracked) {
         if (P.tally) {
             double* h = P.tally + (size_t)sp2 * (P.tally_bins + 2);
@@ -181,16 +184,16 @@ __device__ __forceinline__ void append(const Params& P, Shared& sh, int sp2, con
                 if (b >= 0 && b < P.tally_bins) atomicAdd(&h[b], w);
             }
         }
-        return;
+        return NO_SLOT;
     }
     if (!L.valid) {
         atomicAdd(&sh.bc.flags[FLAG_CAPACITY], 1ull);
-        return;
+        return NO_SLOT;
     }
     ull slot = atomicAdd(L.n, 1ull);
     if (slot >= L.cap) {
         atomicAdd(&sh.bc.flags[FLAG_CAPACITY], 1ull);
-        return;
+        return NO_SLOT;
     }
     bool oob = false;
     double r = setr(L, E, swarm, oob);
@@ -210,6 +213,7 @@ __device__ __forceinline__ void append(const Params& P, Shared& sh, int sp2, con
     L.active[slot] = 1;
     atomicAdd(&sh.bc.work[W_SECONDARIES], 1ull);
     if (main_pass) atomicAdd(&sh.bc.work[W_MAIN_SEC], 1ull);
+    return slot;
 }
 
 // ------------------------------------------------------------------ particle loop
@@ -220,7 +224,9 @@ __device__ __forceinline__ void append(const Params& P, Shared& sh, int sp2, con
 // lanes of a wave stay busy whatever their particles' collision counts.  r is recomputed
 // when a particle is loaded (advance_init!) and droplow!'s cut is applied when it is done.
 // Work items q in [start, total) with stride `stride` index the concatenation of the three
-// species' ranges [rb[s], re[s]).
+// species' ranges [rb[s], re[s]).  The secondaries a lane creates are advanced by that lane
+// right after their parent (up to NCHILD pending per lane; the rest are left to the tail
+// passes, which skip every particle already at tfinal).
 __device__ __forceinline__ void process_range(const Params& P, Shared& sh, const ull* rb, const ull* re, ull start,
                               ull stride, double tfinal, uint32_t step, bool main_pass) {
     const ull n0 = re[0] - rb[0], n1 = re[1] - rb[1], n2 = re[2] - rb[2];
@@ -235,8 +241,38 @@ __device__ __forceinline__ void process_range(const Params& P, Shared& sh, const
     PhiloxStream g;
     g.init(0, 0, 0);
     uint32_t iters = 0, nev = 0;
-    bool active = false, oob = false;
+    bool active = false, oob = false, child = false;
+    int nch = 0;
+    int chsp[NCHILD];
+    ull chidx[NCHILD];
     for (;;) {
+        if (!have && nch > 0) {
+            // a pending secondary of this lane
+            nch--;
+            sp = nch == 0 ? chsp[0] : chsp[1];
+            idx = nch == 0 ? chidx[0] : chidx[1];
+            const SpLite& L = sh.sp[sp];
+            t = L.f[PT_T][idx];
+            trem = tfinal - t;
+            if (L.active[idx] && trem > EPS64) {
+                x = V3{L.f[PT_X][idx], L.f[PT_Y][idx], L.f[PT_Z][idx]};
+                p = V3{L.f[PT_PX][idx], L.f[PT_PY][idx], L.f[PT_PZ][idx]};
+                w = L.f[PT_W][idx];
+                s = L.f[PT_S][idx];
+                uid = L.uid[idx];
+                swarm = L.swarm[idx];
+                if (swarm < 0 || swarm >= P.nswarm) swarm = 0;
+                oob = false;
+                r = setr(L, kinenergy(sp, p), swarm, oob);
+                g.init(P.seed, uid, step);
+                iters = 0;
+                nev = 0;
+                active = true;
+                have = true;
+                child = true;
+            }
+            continue;
+        }
         if (!have) {
             while (q < total) {
                 ull qq = q;
@@ -270,6 +306,7 @@ __device__ __forceinline__ void process_range(const Params& P, Shared& sh, const
                 nev = 0;
                 active = true;
                 have = true;
+                child = !main_pass;
                 break;
             }
             if (!have) break;
@@ -331,12 +368,34 @@ __device__ __forceinline__ void process_range(const Params& P, Shared& sh, const
                             r = setr(L, kinenergy(sp, p), swarm, oob);
                         } else {
                             active = false;
-                            append(P, sh, e.sp0, x, m0, w, t, s0, child_uid(P.seed, uid, step, dend, 0),
-                                   swarm, main_pass);
+                            ull sl = append(P, sh, e.sp0, x, m0, w, t, s0,
+                                            child_uid(P.seed, uid, step, dend, 0), swarm, main_pass);
+                            if (sl != NO_SLOT && nch < NCHILD) {
+                                if (nch == 0) {
+                                    chsp[0] = e.sp0;
+                                    chidx[0] = sl;
+                                } else {
+                                    chsp[1] = e.sp0;
+                                    chidx[1] = sl;
+                                }
+                                nch++;
+                            }
                         }
-                        if (e.ne == 2)
-                            append(P, sh, e.sp1, x, m1, w, t, s1,
-                                   child_uid(P.seed, uid, step, dend, e.keep ? 0 : 1), swarm, main_pass);
+                        if (e.ne == 2) {
+                            ull sl = append(P, sh, e.sp1, x, m1, w, t, s1,
+                                            child_uid(P.seed, uid, step, dend, e.keep ? 0 : 1), swarm,
+                                            main_pass);
+                            if (sl != NO_SLOT && nch < NCHILD) {
+                                if (nch == 0) {
+                                    chsp[0] = e.sp1;
+                                    chidx[0] = sl;
+                                } else {
+                                    chsp[1] = e.sp1;
+                                    chidx[1] = sl;
+                                }
+                                nch++;
+                            }
+                        }
                     }
                 }
             }
@@ -356,7 +415,7 @@ __device__ __forceinline__ void process_range(const Params& P, Shared& sh, const
             if (!active) L.active[idx] = 0;
             if (oob) atomicAdd(&sh.bc.flags[FLAG_RANGE], 1ull);
             if (nev) atomicAdd(&sh.bc.work[W_EVENTS], (ull)nev);
-            atomicAdd(&sh.bc.work[main_pass ? W_PSTEPS : W_TAIL], 1ull);
+            atomicAdd(&sh.bc.work[child ? W_TAIL : W_PSTEPS], 1ull);
             have = false;
         }
     }

@@ -1,0 +1,163 @@
+"""
+    ParticulatorHIP
+
+Julia binding of libparticulator_hip.so (include/particulator_hip.h): a device-resident
+MultiPopulation whose `advance!` runs the HIP step on an MI355X.  Tables are built by
+Particulator.jl itself (`collision_table_from_processes`) and uploaded as they are.
+
+Not exercised in this repository's CI (there is no Julia toolchain in the build image);
+the Python package `particulator_hip` is the tested host of the same C ABI.
+"""
+module ParticulatorHIP
+
+using Particulator
+using StaticArrays
+
+const libpt = joinpath(@__DIR__, "..", "particulator_hip", "libparticulator_hip.so")
+
+# enums of include/particulator_hip.h
+const PT_ELECTRON, PT_PHOTON, PT_POSITRON = 0, 1, 2
+const PROC_KIND = Dict(Particulator.RelativisticCoulomb => 1, Particulator.SeltzerBerger => 2,
+                       Particulator.RBEB => 3, Particulator.Moller => 4, Particulator.Compton => 5,
+                       Particulator.PhotoElectric => 6, Particulator.BetheHeitler => 7,
+                       Particulator.Bhaba => 8, Particulator.PositronAnihilation => 9,
+                       Particulator.KleinNishinaCompton => 10)
+
+struct PtProcDesc
+    kind::Int32; Z::Int32; slot::Int32; N::Int32
+    B::Float64; U::Float64; tcut::Float64; zf::Float64
+    c::NTuple{4, Float64}
+end
+
+struct PtTableDesc
+    nprocs::Int32; order::Int32; nintervals::Int32; nswarm::Int32
+    xmax::Float64; energy_cut::Float64
+    rate::Ptr{Float64}; ratebound::Ptr{Float64}; procs::Ptr{PtProcDesc}
+end
+
+struct PtFieldFn
+    kind::Int32; pad::Int32
+    v::NTuple{3, Float64}; v2::NTuple{3, Float64}
+    z1::Float64; z2::Float64; sx::Float64; sy::Float64; sz::Float64; ez0::Float64
+end
+
+struct PtFieldDesc
+    e::PtFieldFn
+    b::PtFieldFn
+end
+
+struct PtConfig
+    capacity::NTuple{3, UInt64}; seed::UInt64; tracked::NTuple{3, Int32}
+    nswarm::Int32; device::Int32; tally_bins::Int32; tally_lo::Float64; tally_hi::Float64
+end
+
+check(ctx, rc) = rc == 0 || error("particulator_hip: ", unsafe_string(
+    ccall((:pt_last_error, libpt), Cstring, (Ptr{Cvoid},), ctx)))
+
+"""A MultiPopulation resident on one GPU."""
+mutable struct DeviceMultiPopulation
+    ctx::Ptr{Cvoid}
+    keep::Vector{Any}
+end
+
+function DeviceMultiPopulation(; capacity=(1 << 20, 1 << 20, 1 << 20), seed=0, tracked=(1, 1, 1))
+    cfg = Ref(PtConfig(UInt64.(capacity), UInt64(seed), Int32.(tracked), 1, 0, 512,
+                       1e4 * Particulator.co.eV, 1e7 * Particulator.co.eV))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:pt_create, libpt), Cint, (Ref{PtConfig}, Ref{Ptr{Cvoid}}), cfg, h)
+    rc == 0 || error("pt_create failed ($rc)")
+    m = DeviceMultiPopulation(h[], Any[])
+    finalizer(x -> ccall((:pt_destroy, libpt), Cvoid, (Ptr{Cvoid},), x.ctx), m)
+    return m
+end
+
+"""Upload a ChebyshevCollisionTable (its `rate` array is passed as is: (order, L, k+1))."""
+function upload_table!(m::DeviceMultiPopulation, species, tbl, energy_cut)
+    procs = PtProcDesc[]
+    sbslot = 0
+    peslot = 0
+    for p in tbl.proc
+        kind = PROC_KIND[typeof(p).name.wrapper]
+        Z = hasproperty(p, :Z) ? Int32(p.Z) : Int32(0)
+        c = (0.0, 0.0, 0.0, 0.0)
+        slot = Int32(0)
+        if kind == 1
+            c = (1.3413 * p.Z^(-1//3) * Particulator.co.a_0, 0.0, 0.0, 0.0)
+        elseif kind == 7
+            fz = 8 * log(p.Z) / 3
+            c = (p.Z^(1/3), fz, fz + 8 * Particulator._fc(p.Z), 0.0)
+        elseif kind == 2
+            slot = Int32(sbslot); sbslot += 1
+            data = Matrix(p.data)       # (ncum, ne): each column a primary energy
+            check(m.ctx, ccall((:pt_upload_sb, libpt), Cint,
+                               (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Ptr{Float64}, Ptr{Float64}),
+                               m.ctx, slot, p.Z, length(p.log_energy), size(data, 1),
+                               p.log_energy, data))
+        elseif kind == 6
+            slot = Int32(peslot); peslot += 1
+            check(m.ctx, ccall((:pt_upload_pe, libpt), Cint,
+                               (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}),
+                               m.ctx, slot, p.Z, length(p.binding), p.binding))
+        end
+        B = hasproperty(p, :B) ? p.B : 0.0
+        U = hasproperty(p, :U) ? p.U : 0.0
+        N = hasproperty(p, :N) ? Int32(p.N) : Int32(0)
+        tcut = hasproperty(p, :tcut) ? p.tcut : 0.0
+        push!(procs, PtProcDesc(kind, Z, slot, N, B, U, tcut, Float64(Z), c))
+    end
+    rb = Array{Float64}(tbl.ratebound)
+    rate = Array{Float64}(tbl.rate)
+    push!(m.keep, (procs, rb, rate))
+    d = Ref(PtTableDesc(length(procs), size(rate, 1), tbl.b.k, 1, tbl.b.xmax, energy_cut,
+                        pointer(rate), pointer(rb), pointer(procs)))
+    GC.@preserve procs rb rate check(m.ctx, ccall((:pt_upload_table, libpt), Cint,
+                                                  (Ptr{Cvoid}, Cint, Ref{PtTableDesc}), m.ctx, species, d))
+end
+
+fieldfn(f::HomogeneousField) = PtFieldFn(1, 0, Tuple(f.v), (0.0, 0.0, 0.0), 0, 0, 0, 0, 0, 0)
+fieldfn(f::DoubleLayerField) = PtFieldFn(2, 0, Tuple(f.v), (0.0, 0.0, 0.0), f.z1, f.z2, 0, 0, 0, 0)
+fieldfn(f::StepField) = PtFieldFn(3, 0, Tuple(f.v1), Tuple(f.v2), f.z, 0, 0, 0, 0, 0)
+fieldfn(f::ConfinedDoubleLayerField) = PtFieldFn(4, 0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0, 0, f.sx, f.sy, f.sz, f.ez0)
+
+function set_pusher!(m::DeviceMultiPopulation, pusher::RK2Pusher{<:ElectromagneticField})
+    d = Ref(PtFieldDesc(fieldfn(pusher.forcing.e), fieldfn(pusher.forcing.b)))
+    check(m.ctx, ccall((:pt_set_field, libpt), Cint, (Ptr{Cvoid}, Cint, Ref{PtFieldDesc}), m.ctx, 0, d))
+end
+
+"""Append the states of a reference Population (its StructVector columns)."""
+function add!(m::DeviceMultiPopulation, species, parts)
+    n = length(parts)
+    cols = [Float64[p.x[1] for p in parts], Float64[p.x[2] for p in parts], Float64[p.x[3] for p in parts],
+            Float64[p.p[1] for p in parts], Float64[p.p[2] for p in parts], Float64[p.p[3] for p in parts],
+            Float64[p.w for p in parts], Float64[p.t for p in parts], Float64[p.s for p in parts],
+            Float64[p.r for p in parts]]
+    ptrs = [pointer(c) for c in cols]
+    GC.@preserve cols check(m.ctx, ccall((:pt_add_particles, libpt), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Ptr{Float64}}, Ptr{UInt64}, Ptr{Int32}, Csize_t),
+        m.ctx, species, ptrs, C_NULL, C_NULL, n))
+end
+
+Particulator.init!(m::DeviceMultiPopulation) = check(m.ctx, ccall((:pt_init, libpt), Cint, (Ptr{Cvoid},), m.ctx))
+
+"""advance!(mpopl, pusher, tfinal, callback) on the device (the callback's onstep/onoutput
+run on the host in run!)."""
+function Particulator.advance!(m::DeviceMultiPopulation, pusher, tfinal, callback=VoidCallback())
+    check(m.ctx, ccall((:pt_advance, libpt), Cint, (Ptr{Cvoid}, Cdouble), m.ctx, tfinal))
+end
+
+"""droplow! of every species: the cut is applied in the step; this frees the slots."""
+droplow_all!(m::DeviceMultiPopulation) = check(m.ctx, ccall((:pt_compact, libpt), Cint, (Ptr{Cvoid}, Cint), m.ctx, -1))
+
+function run!(m::DeviceMultiPopulation, pusher, tfinal, dt, callback=VoidCallback(); compact_every=8)
+    set_pusher!(m, pusher)
+    t = 0.0
+    i = 0
+    while t < tfinal
+        Particulator.advance!(m, pusher, t + dt, callback)
+        i += 1
+        i % compact_every == 0 && droplow_all!(m)
+        t += dt
+    end
+end
+
+end
