@@ -153,6 +153,7 @@ typedef struct {
     double tail_ms;
     double compact_ms;
     uint64_t main_secondaries;   /* particles appended by the main advance kernel */
+    double graph_ms;             /* summed duration of graph replays (pt_run_graph) */
 } pt_work;
 
 /* ---- lifecycle ---------------------------------------------------------------- */
@@ -210,6 +211,43 @@ int pt_run(pt_ctx* ctx, double t0, double dt, int nsteps, int compact_every, int
 /* repack! (src/population.jl:229-259): fill holes below the active count with active
  * particles from the end (not order preserving, as in the reference). */
 int pt_compact(pt_ctx* ctx, int species);
+/* ---- launch shape, captured runs, autotuning ---------------------------------------
+ * The particle results do not depend on any of these: every random stream is keyed by
+ * (seed, particle uid, step), so launch shape and compaction cadence change only where a
+ * particle sits in its population, not its history. */
+typedef struct {
+    int32_t block;          /* threads per workgroup of the advance kernel: 128, 256 or 512 */
+    int32_t grid_mult;      /* workgroups launched = resident workgroups x grid_mult (1..16) */
+    int32_t compact_every;  /* steps between compactions in pt_run_graph (>= 1) */
+    int32_t pad;
+} pt_launch;
+/* No reference counterpart (the reference's @batch threads split the loop statically,
+ * src/mixed_population.jl:61). */
+int pt_set_launch(pt_ctx* ctx, const pt_launch* launch);
+int pt_get_launch(pt_ctx* ctx, pt_launch* launch);
+/* run! (src/run.jl:1-29) with a fixed dt and no host callbacks: compact_every steps
+ * (snapshot, advance, secondary passes, clock tick) and one compaction are captured as a
+ * hipGraph and replayed; the step clock lives on the device (tfinal = t + dt, t = tfinal),
+ * so the times are those of pt_run.  Steps left over (nsteps not a multiple of
+ * compact_every) run as a shorter graph without compaction.  Errors are checked every
+ * check_every steps (rounded up to whole replays). */
+int pt_run_graph(pt_ctx* ctx, double t0, double dt, int nsteps, int check_every);
+typedef struct {
+    pt_launch launch;
+    double rate;            /* particle-steps/s measured over the trial */
+} pt_tune_trial;
+/* Autotuner (no reference counterpart): times trial_steps steps of the live run for each
+ * advance-kernel shape (block 128/256/512 x grid_mult 1/2/4), then 16 steps for each
+ * compaction cadence 1/2/4/8/16 with the fastest shape, and sets the fastest as the
+ * context's launch.  The trial steps advance the run from t0 by dt; *t_end is the time
+ * reached.  trials[] receives up to max_trials measurements. */
+int pt_autotune(pt_ctx* ctx, double t0, double dt, int trial_steps, pt_tune_trial* trials,
+                int max_trials, int* ntrials, pt_launch* best, double* t_end);
+/* nactives / weight / meanenergy per swarm (src/population.jl:89-223 applied to each of
+ * the independent populations the reference scripts run, scripts/swarm.jl:63-99):
+ * out[4*i + 0..3] = active count, weight, sum of w*E (J), max E (J) of swarm i. */
+int pt_swarm_stats(pt_ctx* ctx, int species, int nswarm, double* out);
+
 /* population statistics (src/population.jl:89-223) over active particles */
 int pt_stats(pt_ctx* ctx, int species, pt_pop_stats* out);
 /* CollisionCounter (src/callback.jl:118-141): out[0] null events, out[1+j] process j in

@@ -52,6 +52,16 @@ enum {
 };
 constexpr int NCOUNT = MAX_PROCS + 1;
 
+// Step clock in device memory.  pt_advance writes tfinal and the step number; a captured
+// run derives them on the device (tfinal = t + dt, then t = tfinal and step + 1 at the end
+// of the step: the same arithmetic as the host loop t += dt), so replays of one graph
+// advance the time.
+struct Clock {
+    double t, dt, tfinal;
+    uint32_t step, pad;
+    ull cursor;  // work items claimed in the main pass
+};
+
 struct Params {
     SpeciesDev sp[PT_NSPECIES];
     DevTable tab[PT_NSPECIES];
@@ -68,6 +78,7 @@ struct Params {
     int tally_bins;
     double tally_lo, tally_hi;
     ull* range;  // [3][2]
+    Clock* clock;
     int lds_rate_off[PT_NSPECIES];
     int lds_rb_off[PT_NSPECIES];
     int lds_doubles;
@@ -220,18 +231,52 @@ __device__ __forceinline__ ull append(const Params& P, Shared& sh, int sp2, cons
 // The body of advance1!'s @batch loop (src/mixed_population.jl:61-87), with
 // do_one_collision! (src/collisions.jl:142-199) and apply! (:83-133) inlined, run as an
 // event loop: each iteration advances a lane's particle by one push (and at most one
-// collision); a lane whose particle has reached tfinal loads its next particle at once, so
+// collision); a lane whose particle has reached tfinal takes its next particle at once, so
 // lanes of a wave stay busy whatever their particles' collision counts.  r is recomputed
 // when a particle is loaded (advance_init!) and droplow!'s cut is applied when it is done.
-// Work items q in [start, total) with stride `stride` index the concatenation of the three
-// species' ranges [rb[s], re[s]).  The secondaries a lane creates are advanced by that lane
-// right after their parent (up to NCHILD pending per lane; the rest are left to the tail
-// passes, which skip every particle already at tfinal).
-__device__ __forceinline__ void process_range(const Params& P, Shared& sh, const ull* rb, const ull* re, ull start,
-                              ull stride, double tfinal, uint32_t step, bool main_pass) {
+//
+// Work items q in [0, total) index the concatenation of the three species' ranges
+// [rb[s], re[s]).  A wave claims chunks of CHUNK consecutive items from `cursor` (a device
+// counter for the main pass, an LDS counter for the tail passes) and hands them out in
+// order to its free lanes (ballot rank), so the particles a wave holds at any moment lie
+// in a short window of consecutive slots: refills read whole cache lines and write-backs
+// of neighbouring lanes meet in L2.  The secondaries a lane creates are advanced by that
+// lane right after their parent (up to NCHILD pending per lane; the rest are left to the
+// tail passes, which skip every particle already at tfinal).
+constexpr int CHUNK = 256;
+
+struct Loaded {
+    V3 x, p;
+    double w, t, s;
+    uint64_t uid;
+    int swarm;
+    bool act;
+};
+
+// one round trip: every column of slot idx is read before any is tested
+__device__ __forceinline__ Loaded load_slot(const SpLite& L, ull idx) {
+    Loaded a;
+    a.act = L.active[idx] != 0;
+    a.t = L.f[PT_T][idx];
+    a.x = V3{L.f[PT_X][idx], L.f[PT_Y][idx], L.f[PT_Z][idx]};
+    a.p = V3{L.f[PT_PX][idx], L.f[PT_PY][idx], L.f[PT_PZ][idx]};
+    a.w = L.f[PT_W][idx];
+    a.s = L.f[PT_S][idx];
+    a.uid = L.uid[idx];
+    a.swarm = L.swarm[idx];
+    return a;
+}
+
+__device__ __forceinline__ void process_range(const Params& P, Shared& sh, const ull* rb, const ull* re,
+                                              ull* cursor, double tfinal, uint32_t step, bool main_pass) {
     const ull n0 = re[0] - rb[0], n1 = re[1] - rb[1], n2 = re[2] - rb[2];
     const ull total = n0 + n1 + n2;
-    ull q = start;
+    const int lane = threadIdx.x & 63;
+    const uint64_t ltmask = lane ? (~0ull >> (64 - lane)) : 0ull;
+    // wave-uniform chunk state
+    ull cbeg = 0, cend = 0;
+    bool drained = false;
+    // lane state
     bool have = false;
     int sp = 0, swarm = 0;
     ull idx = 0;
@@ -246,24 +291,26 @@ __device__ __forceinline__ void process_range(const Params& P, Shared& sh, const
     int chsp[NCHILD];
     ull chidx[NCHILD];
     for (;;) {
-        if (!have && nch > 0) {
-            // a pending secondary of this lane
+        // 1. pending secondaries of this lane
+        while (!have && nch > 0) {
             nch--;
-            sp = nch == 0 ? chsp[0] : chsp[1];
-            idx = nch == 0 ? chidx[0] : chidx[1];
-            const SpLite& L = sh.sp[sp];
-            t = L.f[PT_T][idx];
-            trem = tfinal - t;
-            if (L.active[idx] && trem > EPS64) {
-                x = V3{L.f[PT_X][idx], L.f[PT_Y][idx], L.f[PT_Z][idx]};
-                p = V3{L.f[PT_PX][idx], L.f[PT_PY][idx], L.f[PT_PZ][idx]};
-                w = L.f[PT_W][idx];
-                s = L.f[PT_S][idx];
-                uid = L.uid[idx];
-                swarm = L.swarm[idx];
-                if (swarm < 0 || swarm >= P.nswarm) swarm = 0;
+            int csp = nch == 0 ? chsp[0] : chsp[1];
+            ull cidx = nch == 0 ? chidx[0] : chidx[1];
+            Loaded a = load_slot(sh.sp[csp], cidx);
+            double cr = tfinal - a.t;
+            if (a.act && cr > EPS64) {
+                sp = csp;
+                idx = cidx;
+                x = a.x;
+                p = a.p;
+                w = a.w;
+                t = a.t;
+                s = a.s;
+                uid = a.uid;
+                swarm = (a.swarm < 0 || a.swarm >= P.nswarm) ? 0 : a.swarm;
+                trem = cr;
                 oob = false;
-                r = setr(L, kinenergy(sp, p), swarm, oob);
+                r = setr(sh.sp[sp], kinenergy(sp, p), swarm, oob);
                 g.init(P.seed, uid, step);
                 iters = 0;
                 nev = 0;
@@ -271,46 +318,68 @@ __device__ __forceinline__ void process_range(const Params& P, Shared& sh, const
                 have = true;
                 child = true;
             }
-            continue;
         }
-        if (!have) {
-            while (q < total) {
-                ull qq = q;
-                q += stride;
-                if (qq < n0) {
-                    sp = 0;
-                    idx = rb[0] + qq;
-                } else if (qq < n0 + n1) {
-                    sp = 1;
-                    idx = rb[1] + (qq - n0);
-                } else {
-                    sp = 2;
-                    idx = rb[2] + (qq - n0 - n1);
+        // 2. free lanes take the next items of the wave's chunk, in lane order
+        uint64_t need = __ballot(!have);
+        while (need && !drained) {
+            if (cbeg >= cend) {
+                int fl = __ffsll((unsigned long long)__ballot(1)) - 1;
+                ull b = 0;
+                if (lane == fl) b = atomicAdd(cursor, (ull)CHUNK);
+                b = __shfl(b, fl);
+                if (b >= total) {
+                    drained = true;
+                    break;
                 }
-                const SpLite& L = sh.sp[sp];
-                if (!L.active[idx]) continue;
-                t = L.f[PT_T][idx];
-                trem = tfinal - t;
-                if (!(trem > EPS64)) continue;
-                x = V3{L.f[PT_X][idx], L.f[PT_Y][idx], L.f[PT_Z][idx]};
-                p = V3{L.f[PT_PX][idx], L.f[PT_PY][idx], L.f[PT_PZ][idx]};
-                w = L.f[PT_W][idx];
-                s = L.f[PT_S][idx];
-                uid = L.uid[idx];
-                swarm = L.swarm[idx];
-                if (swarm < 0 || swarm >= P.nswarm) swarm = 0;
-                oob = false;
-                r = setr(L, kinenergy(sp, p), swarm, oob);
-                g.init(P.seed, uid, step);
-                iters = 0;
-                nev = 0;
-                active = true;
-                have = true;
-                child = !main_pass;
-                break;
+                cbeg = b;
+                cend = b + CHUNK < total ? b + CHUNK : total;
             }
-            if (!have) break;
+            ull q = cbeg + (ull)__popcll(need & ltmask);
+            ull cnt = (ull)__popcll(need);
+            bool mine = !have && q < cend;
+            cbeg = cbeg + cnt < cend ? cbeg + cnt : cend;
+            if (mine) {
+                int qs;
+                ull qi;
+                if (q < n0) {
+                    qs = 0;
+                    qi = rb[0] + q;
+                } else if (q < n0 + n1) {
+                    qs = 1;
+                    qi = rb[1] + (q - n0);
+                } else {
+                    qs = 2;
+                    qi = rb[2] + (q - n0 - n1);
+                }
+                Loaded a = load_slot(sh.sp[qs], qi);
+                double cr = tfinal - a.t;
+                if (a.act && cr > EPS64) {
+                    sp = qs;
+                    idx = qi;
+                    x = a.x;
+                    p = a.p;
+                    w = a.w;
+                    t = a.t;
+                    s = a.s;
+                    uid = a.uid;
+                    swarm = (a.swarm < 0 || a.swarm >= P.nswarm) ? 0 : a.swarm;
+                    trem = cr;
+                    oob = false;
+                    r = setr(sh.sp[sp], kinenergy(sp, p), swarm, oob);
+                    g.init(P.seed, uid, step);
+                    iters = 0;
+                    nev = 0;
+                    active = true;
+                    have = true;
+                    child = !main_pass;
+                }
+            }
+            need = __ballot(!have);
         }
+        // 3. the wave leaves when its items are claimed and no lane holds a particle
+        if (drained && __ballot(have) == 0) break;
+        if (!have) continue;
+        // 4. one push (and at most one collision) of this lane's particle
         const SpLite& L = sh.sp[sp];
         if (++iters > ITER_LIMIT || g.exhausted()) {
             atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
@@ -422,7 +491,7 @@ __device__ __forceinline__ void process_range(const Params& P, Shared& sh, const
 }
 
 // ------------------------------------------------------------------ step kernels
-__global__ void k_snapshot(const Params* __restrict__ Pp) {
+__global__ void k_snapshot(const Params* __restrict__ Pp, int clocked, double tfinal, uint32_t step) {
     const Params& P = *Pp;
     int s = threadIdx.x;
     if (s < PT_NSPECIES) {
@@ -431,10 +500,28 @@ __global__ void k_snapshot(const Params* __restrict__ Pp) {
         P.range[2 * s] = 0;
         P.range[2 * s + 1] = n;
     }
+    if (s == 0) {
+        Clock& C = *P.clock;
+        if (clocked) {
+            C.tfinal = C.t + C.dt;
+        } else {
+            C.tfinal = tfinal;
+            C.step = step;
+        }
+        C.cursor = 0;
+    }
 }
 
-__global__ void __launch_bounds__(256) k_advance(const Params* __restrict__ Pp, double tfinal,
-                                                 uint32_t step) {
+__global__ void k_tick(const Params* __restrict__ Pp) {
+    Clock& C = *Pp->clock;
+    if (threadIdx.x == 0) {
+        C.t = C.tfinal;
+        C.step += 1;
+    }
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_advance(const Params* __restrict__ Pp) {
     extern __shared__ double lds[];
     __shared__ Shared sh;
     const Params& P = *Pp;
@@ -444,21 +531,23 @@ __global__ void __launch_bounds__(256) k_advance(const Params* __restrict__ Pp, 
         rb[s] = P.range[2 * s];
         re[s] = P.range[2 * s + 1];
     }
-    process_range(P, sh, rb, re, (ull)blockIdx.x * blockDim.x + threadIdx.x,
-                  (ull)gridDim.x * blockDim.x, tfinal, step, true);
+    const Clock& C = *P.clock;
+    process_range(P, sh, rb, re, &P.clock->cursor, C.tfinal, C.step, true);
     flush_counters(P, &sh.bc);
 }
 
 // Secondaries born in this step: one workgroup repeats passes over [begin, n) until a pass
 // appends nothing (the `while n > 0` loop of advance!, src/mixed_population.jl:43-46).
-__global__ void __launch_bounds__(512) k_tail(const Params* __restrict__ Pp, double tfinal,
-                                              uint32_t step, int maxpasses) {
+__global__ void __launch_bounds__(512) k_tail(const Params* __restrict__ Pp, int maxpasses) {
     extern __shared__ double lds[];
     __shared__ Shared sh;
     __shared__ ull beg[PT_NSPECIES], end[PT_NSPECIES];
+    __shared__ ull cur;
     __shared__ int done;
     const Params& P = *Pp;
     stage(P, lds, sh);
+    const double tfinal = P.clock->tfinal;
+    const uint32_t step = P.clock->step;
     if (threadIdx.x == 0)
         for (int s = 0; s < PT_NSPECIES; s++) beg[s] = P.range[2 * s + 1];
     if (threadIdx.x == 0) done = 0;
@@ -467,6 +556,7 @@ __global__ void __launch_bounds__(512) k_tail(const Params* __restrict__ Pp, dou
         __syncthreads();
         if (threadIdx.x == 0) {
             done = 1;
+            cur = 0;
             for (int s = 0; s < PT_NSPECIES; s++) {
                 ull n = __hip_atomic_load(P.sp[s].n, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
                 if (n > P.sp[s].cap) n = P.sp[s].cap;
@@ -481,7 +571,7 @@ __global__ void __launch_bounds__(512) k_tail(const Params* __restrict__ Pp, dou
             rb[s] = beg[s];
             re[s] = end[s];
         }
-        process_range(P, sh, rb, re, threadIdx.x, blockDim.x, tfinal, step, false);
+        process_range(P, sh, rb, re, &cur, tfinal, step, false);
         __threadfence();
         __syncthreads();
         if (threadIdx.x == 0)
@@ -551,8 +641,12 @@ __global__ void __launch_bounds__(256) k_init_particles(const Params* __restrict
 // ------------------------------------------------------------------ compaction
 constexpr int CH = 4096;  // elements per chunk (one 256-thread block)
 
-__global__ void __launch_bounds__(256) k_cmp_count(const uint8_t* __restrict__ active, ull n,
+__global__ void __launch_bounds__(256) k_cmp_count(const uint8_t* __restrict__ active,
+                                                   const ull* __restrict__ ncounter, ull cap,
                                                    unsigned* __restrict__ cnt) {
+    ull n = *ncounter;
+    if (n > cap) n = cap;
+    if ((ull)blockIdx.x * CH >= n) return;
     __shared__ unsigned part[4];
     ull c0 = (ull)blockIdx.x * CH;
     unsigned local = 0;
@@ -569,11 +663,15 @@ __global__ void __launch_bounds__(256) k_cmp_count(const uint8_t* __restrict__ a
 
 // exclusive scan of chunk counts; holes below A and tail actives at/above A per chunk.
 // info[0] = A (active count), info[1] = number of moves, info[2] = old n.
-__global__ void __launch_bounds__(1024) k_cmp_scan(const uint8_t* __restrict__ active, ull n,
-                                                   const unsigned* __restrict__ cnt, int nchunks,
+__global__ void __launch_bounds__(1024) k_cmp_scan(const uint8_t* __restrict__ active,
+                                                   ull* __restrict__ ncounter, ull cap,
+                                                   const unsigned* __restrict__ cnt,
                                                    unsigned* __restrict__ hole_pre,
                                                    unsigned* __restrict__ tail_pre,
-                                                   ull* __restrict__ info, ull* __restrict__ ncounter) {
+                                                   ull* __restrict__ info) {
+    ull n = *ncounter;
+    if (n > cap) n = cap;
+    const int nchunks = (int)((n + CH - 1) / CH);
     __shared__ ull wsum[16];
     __shared__ ull tot_s;
     __shared__ unsigned act_low_s;
@@ -652,6 +750,7 @@ __global__ void __launch_bounds__(1024) k_cmp_scan(const uint8_t* __restrict__ a
         hrun += h;
         trun += tl;
     }
+    __syncthreads();  // every thread has read *ncounter
     if (tid == (int)blockDim.x - 1) {
         info[0] = A;
         info[1] = sh[tid];  // total holes == total tail actives
@@ -661,12 +760,14 @@ __global__ void __launch_bounds__(1024) k_cmp_scan(const uint8_t* __restrict__ a
 }
 
 // positions of the k-th hole and the k-th tail active (ballot prefix sums, chunk order)
-__global__ void __launch_bounds__(256) k_cmp_fill(const uint8_t* __restrict__ active, ull n,
+__global__ void __launch_bounds__(256) k_cmp_fill(const uint8_t* __restrict__ active,
                                                   const ull* __restrict__ info,
                                                   const unsigned* __restrict__ hole_pre,
                                                   const unsigned* __restrict__ tail_pre,
                                                   unsigned* __restrict__ hpos,
                                                   unsigned* __restrict__ tpos) {
+    const ull n = info[2];
+    if ((ull)blockIdx.x * CH >= n) return;
     __shared__ unsigned wh[4], wt[4];
     __shared__ unsigned runh, runt;
     ull A = info[0];
@@ -758,6 +859,58 @@ __global__ void __launch_bounds__(256) k_stats_partial(const Params* __restrict_
         __syncthreads();
     }
     if (threadIdx.x < NSTAT) part[blockIdx.x * NSTAT + threadIdx.x] = red[threadIdx.x][0];
+}
+
+
+// per-swarm sums over active particles: count, w, w*E and max E of each swarm.  With up
+// to SWARM_LDS swarms a workgroup accumulates in LDS and adds its partial sums once;
+// beyond that it adds into global memory directly.  out[4*i + 3] holds max E as the bit
+// pattern of a non-negative double (ordered like the value).
+constexpr int SWARM_LDS = 1024;
+__global__ void __launch_bounds__(256) k_swarm_stats(const Params* __restrict__ Pp, int sp, int nswarm,
+                                                     double* __restrict__ out) {
+    __shared__ double acc[3 * SWARM_LDS];
+    __shared__ ull emax[SWARM_LDS];
+    const Params& P = *Pp;
+    const SpeciesDev& S = P.sp[sp];
+    const bool local = nswarm <= SWARM_LDS;
+    if (local) {
+        for (int q = threadIdx.x; q < 3 * nswarm; q += blockDim.x) acc[q] = 0.0;
+        for (int q = threadIdx.x; q < nswarm; q += blockDim.x) emax[q] = 0ull;
+        __syncthreads();
+    }
+    ull n = *S.n;
+    if (n > S.cap) n = S.cap;
+    for (ull i = (ull)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ull)gridDim.x * blockDim.x) {
+        if (!S.active[i]) continue;
+        int sw = S.swarm[i];
+        if (sw < 0 || sw >= nswarm) continue;
+        V3 p{S.f[PT_PX][i], S.f[PT_PY][i], S.f[PT_PZ][i]};
+        double w = S.f[PT_W][i];
+        double E = kinenergy(sp, p);
+        ull eb = (ull)__double_as_longlong(E > 0.0 ? E : 0.0);
+        if (local) {
+            atomicAdd(&acc[3 * sw], 1.0);
+            atomicAdd(&acc[3 * sw + 1], w);
+            atomicAdd(&acc[3 * sw + 2], w * E);
+            atomicMax(&emax[sw], eb);
+        } else {
+            atomicAdd(&out[4 * sw], 1.0);
+            atomicAdd(&out[4 * sw + 1], w);
+            atomicAdd(&out[4 * sw + 2], w * E);
+            atomicMax((ull*)&out[4 * sw + 3], eb);
+        }
+    }
+    if (local) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < nswarm; q += blockDim.x) {
+            if (acc[3 * q] == 0.0) continue;
+            atomicAdd(&out[4 * q], acc[3 * q]);
+            atomicAdd(&out[4 * q + 1], acc[3 * q + 1]);
+            atomicAdd(&out[4 * q + 2], acc[3 * q + 2]);
+            atomicMax((ull*)&out[4 * q + 3], emax[q]);
+        }
+    }
 }
 
 // ------------------------------------------------------------------ debug kernels
@@ -886,6 +1039,11 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+struct Span {
+    hipEvent_t a, b;
+    int kind;
+};
+
 struct pt_ctx {
     pt_config cfg{};
     int device = 0;
@@ -923,13 +1081,26 @@ struct pt_ctx {
     double* dtally = nullptr;
     uint32_t step = 0;
     uint64_t next_uid = 0;
-    int grid = 0;
+    int grid = 0;         // workgroups of the grid-stride utility kernels
+    int ncu = 0;          // compute units
     int lds_bytes = 0;
+    Clock* dclock = nullptr;
+    pt_launch launch{256, 1, 8, 0};
+    int resident[3] = {0, 0, 0};  // resident advance workgroups per CU for block 128/256/512
+    int resident_lds = -1;        // LDS size the residency was computed for
+    int stat_cap = 0;             // swarms the per-swarm statistics buffer holds
+    double* swarm_part = nullptr;
+    // captured runs: one graph per (steps, compaction) shape
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int gsteps[2] = {0, 0};
+    pt_launch glaunch[2] = {};
+    int glds[2] = {0, 0};
     bool profile = false;
     hipEvent_t ev[8] = {};
-    std::vector<hipEvent_t> evq;  // (begin, middle, end) per profiled step, read lazily
+    std::vector<hipEvent_t> evq;  // event pool of the profiling spans
     size_t evq_used = 0;
-    double adv_ms = 0.0, tail_ms = 0.0, cmp_ms = 0.0;
+    std::vector<Span> spans;      // recorded, not yet read
+    double adv_ms = 0.0, tail_ms = 0.0, cmp_ms = 0.0, graph_ms = 0.0;
     ull adv_launches = 0;
     std::string err;
 };
@@ -1004,6 +1175,8 @@ void pt_destroy(pt_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& g : c->gexec)
+        if (g) hipGraphExecDestroy(g);
     for (void* p : c->allocs) hipFree(p);
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
@@ -1070,6 +1243,8 @@ int pt_create(const pt_config* cfg, pt_ctx** out) {
     TRY(dalloc(c, (void**)&c->dflags, sizeof(ull) * NFLAGS));
     TRY(dalloc(c, (void**)&c->dwork, sizeof(ull) * NWORK));
     TRY(dalloc(c, (void**)&c->drange, sizeof(ull) * 2 * PT_NSPECIES));
+    TRY(dalloc(c, (void**)&c->dclock, sizeof(Clock)));
+    HIPCHK(c, hipMemsetAsync(c->dclock, 0, sizeof(Clock), c->stream));
     HIPCHK(c, hipMemsetAsync(c->dcounts, 0, sizeof(ull) * PT_NSPECIES * NCOUNT, c->stream));
     HIPCHK(c, hipMemsetAsync(c->dflags, 0, sizeof(ull) * NFLAGS, c->stream));
     HIPCHK(c, hipMemsetAsync(c->dwork, 0, sizeof(ull) * NWORK, c->stream));
@@ -1093,6 +1268,7 @@ int pt_create(const pt_config* cfg, pt_ctx** out) {
     P.flags = c->dflags;
     P.work = c->dwork;
     P.range = c->drange;
+    P.clock = c->dclock;
     // compaction scratch
     c->cchunks = (maxcap + CH - 1) / CH + 1;
     TRY(dalloc(c, (void**)&c->ccnt, sizeof(unsigned) * c->cchunks));
@@ -1106,6 +1282,7 @@ int pt_create(const pt_config* cfg, pt_ctx** out) {
     TRY(dalloc(c, (void**)&c->dp, sizeof(Params)));
     hipDeviceProp_t prop;
     HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    c->ncu = prop.multiProcessorCount;
     c->grid = prop.multiProcessorCount * 8;
     c->dirty = true;
     TRY(sync_params(c));
@@ -1402,18 +1579,27 @@ int pt_init(pt_ctx* c) {
     return PT_OK;
 }
 
-// accumulate the durations of the profiled steps recorded so far
+// ---- profiling spans: pairs of events on the context's stream, read lazily
+enum { SPAN_ADV = 0, SPAN_TAIL = 1, SPAN_CMP = 2, SPAN_GRAPH = 3 };
+
 static int drain_events(pt_ctx* c) {
-    if (c->evq_used == 0) return PT_OK;
-    HIPCHK(c, hipEventSynchronize(c->evq[c->evq_used - 1]));
-    for (size_t q = 0; q + 3 <= c->evq_used; q += 3) {
-        float a = 0, b = 0;
-        HIPCHK(c, hipEventElapsedTime(&a, c->evq[q], c->evq[q + 1]));
-        HIPCHK(c, hipEventElapsedTime(&b, c->evq[q + 1], c->evq[q + 2]));
-        c->adv_ms += a;
-        c->tail_ms += b;
-        c->adv_launches++;
+    if (c->spans.empty()) return PT_OK;
+    HIPCHK(c, hipEventSynchronize(c->spans.back().b));
+    for (const Span& s : c->spans) {
+        float a = 0;
+        HIPCHK(c, hipEventElapsedTime(&a, s.a, s.b));
+        if (s.kind == SPAN_ADV) {
+            c->adv_ms += a;
+            c->adv_launches++;
+        } else if (s.kind == SPAN_TAIL) {
+            c->tail_ms += a;
+        } else if (s.kind == SPAN_CMP) {
+            c->cmp_ms += a;
+        } else {
+            c->graph_ms += a;
+        }
     }
+    c->spans.clear();
     c->evq_used = 0;
     return PT_OK;
 }
@@ -1428,25 +1614,74 @@ static int next_event(pt_ctx* c, hipEvent_t* e) {
     return PT_OK;
 }
 
-static int launch_step(pt_ctx* c, double tfinal) {
-    hipLaunchKernelGGL(k_snapshot, dim3(1), dim3(64), 0, c->stream, c->dp);
-    HIPCHK(c, hipGetLastError());
-    ull est = c->hn[0] + c->hn[1] + c->hn[2];
-    int blocks = (int)std::min<ull>((ull)c->grid, std::max<ull>(1, (est + 255) / 256));
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+// record an event that opens (or, with `kind`, closes) a span
+static int mark(pt_ctx* c, hipEvent_t* e) {
     int rc;
-    if (c->profile) {
-        if (c->evq_used + 3 > 3 * 1024 && (rc = drain_events(c))) return rc;
-        if ((rc = next_event(c, &e0)) || (rc = next_event(c, &e1)) || (rc = next_event(c, &e2))) return rc;
-        HIPCHK(c, hipEventRecord(e0, c->stream));
+    if (c->evq_used + 2 > 4096 && (rc = drain_events(c))) return rc;
+    if ((rc = next_event(c, e))) return rc;
+    HIPCHK(c, hipEventRecord(*e, c->stream));
+    return PT_OK;
+}
+
+// ---- launch shape
+static int block_index(int bs) { return bs == 128 ? 0 : bs == 512 ? 2 : 1; }
+
+static int ensure_residency(pt_ctx* c) {
+    if (c->resident_lds == c->lds_bytes) return PT_OK;
+    int nb = 0;
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_advance<128>, 128, c->lds_bytes));
+    c->resident[0] = std::max(nb, 1);
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_advance<256>, 256, c->lds_bytes));
+    c->resident[1] = std::max(nb, 1);
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_advance<512>, 512, c->lds_bytes));
+    c->resident[2] = std::max(nb, 1);
+    c->resident_lds = c->lds_bytes;
+    return PT_OK;
+}
+
+// full: launch every resident workgroup x grid_mult (captured runs, where the population
+// is not known on the host); otherwise at most what the last known population needs
+static int launch_advance(pt_ctx* c, bool full) {
+    int rc = ensure_residency(c);
+    if (rc) return rc;
+    const int bs = c->launch.block;
+    ull blocks = (ull)c->resident[block_index(bs)] * c->ncu * std::max(1, c->launch.grid_mult);
+    if (!full) {
+        ull est = c->hn[0] + c->hn[1] + c->hn[2];
+        blocks = std::min<ull>(blocks, std::max<ull>((ull)c->ncu, (2 * est + bs - 1) / bs));
     }
-    hipLaunchKernelGGL(k_advance, dim3(blocks), dim3(256), c->lds_bytes, c->stream, c->dp, tfinal, c->step);
+    dim3 g((unsigned)std::max<ull>(blocks, 1));
+    if (bs == 128) hipLaunchKernelGGL(k_advance<128>, g, dim3(128), c->lds_bytes, c->stream, c->dp);
+    else if (bs == 512) hipLaunchKernelGGL(k_advance<512>, g, dim3(512), c->lds_bytes, c->stream, c->dp);
+    else hipLaunchKernelGGL(k_advance<256>, g, dim3(256), c->lds_bytes, c->stream, c->dp);
     HIPCHK(c, hipGetLastError());
-    if (c->profile) HIPCHK(c, hipEventRecord(e1, c->stream));
-    hipLaunchKernelGGL(k_tail, dim3(1), dim3(512), c->lds_bytes, c->stream, c->dp, tfinal, c->step, 100000);
+    return PT_OK;
+}
+
+// one step.  clocked: tfinal and the step index come from the device clock (captured
+// runs); otherwise they are given and the host counts steps.
+static int launch_step(pt_ctx* c, bool clocked, double tfinal) {
+    hipLaunchKernelGGL(k_snapshot, dim3(1), dim3(64), 0, c->stream, c->dp, clocked ? 1 : 0, tfinal, c->step);
     HIPCHK(c, hipGetLastError());
-    if (c->profile) HIPCHK(c, hipEventRecord(e2, c->stream));
-    c->step++;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    const bool prof = c->profile && !clocked;
+    int rc;
+    if (prof && (rc = mark(c, &e0))) return rc;
+    if ((rc = launch_advance(c, clocked))) return rc;
+    if (prof && (rc = mark(c, &e1))) return rc;
+    hipLaunchKernelGGL(k_tail, dim3(1), dim3(512), c->lds_bytes, c->stream, c->dp, 100000);
+    HIPCHK(c, hipGetLastError());
+    if (prof) {
+        if ((rc = mark(c, &e2))) return rc;
+        c->spans.push_back(Span{e0, e1, SPAN_ADV});
+        c->spans.push_back(Span{e1, e2, SPAN_TAIL});
+    }
+    if (clocked) {
+        hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c->stream, c->dp);
+        HIPCHK(c, hipGetLastError());
+    } else {
+        c->step++;
+    }
     return PT_OK;
 }
 
@@ -1477,29 +1712,45 @@ int pt_advance(pt_ctx* c, double tfinal) {
     if (rc) return rc;
     rc = sync_params(c);
     if (rc) return rc;
-    rc = launch_step(c, tfinal);
+    rc = launch_step(c, false, tfinal);
     if (rc) return rc;
     rc = read_counts(c);
     if (rc) return rc;
     return check_flags(c, nullptr);
 }
 
+// compaction of one species, entirely on the device (the count is read there), so that it
+// can be captured; the grid covers the capacity and chunks past the count exit at once
 static int compact_species(pt_ctx* c, int sp) {
-    ull n = c->hn[sp];
-    if (n == 0) return PT_OK;
-    int nch = (int)((n + CH - 1) / CH);
     SpeciesDev& S = c->hp.sp[sp];
-    hipLaunchKernelGGL(k_cmp_count, dim3(nch), dim3(256), 0, c->stream, S.active, n, c->ccnt);
+    const ull cap = S.cap;
+    if (cap == 0) return PT_OK;
+    int nch = (int)((cap + CH - 1) / CH);
+    hipLaunchKernelGGL(k_cmp_count, dim3(nch), dim3(256), 0, c->stream, S.active, S.n, cap, c->ccnt);
     HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(k_cmp_scan, dim3(1), dim3(1024), 0, c->stream, S.active, n, c->ccnt, nch, c->chole,
-                       c->ctail, c->cinfo, c->dn + sp);
+    hipLaunchKernelGGL(k_cmp_scan, dim3(1), dim3(1024), 0, c->stream, S.active, S.n, cap, c->ccnt, c->chole,
+                       c->ctail, c->cinfo);
     HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(k_cmp_fill, dim3(nch), dim3(256), 0, c->stream, S.active, n, c->cinfo, c->chole,
+    hipLaunchKernelGGL(k_cmp_fill, dim3(nch), dim3(256), 0, c->stream, S.active, c->cinfo, c->chole,
                        c->ctail, c->hpos, c->tpos);
     HIPCHK(c, hipGetLastError());
-    int mb = (int)std::min<ull>((ull)c->grid, std::max<ull>(1, (n / 2 + 255) / 256));
-    hipLaunchKernelGGL(k_cmp_move, dim3(mb), dim3(256), 0, c->stream, S, c->cinfo, c->hpos, c->tpos, c->dwork);
+    hipLaunchKernelGGL(k_cmp_move, dim3(c->grid), dim3(256), 0, c->stream, S, c->cinfo, c->hpos, c->tpos, c->dwork);
     HIPCHK(c, hipGetLastError());
+    return PT_OK;
+}
+
+static int compact_all(pt_ctx* c, int sp, bool timed) {
+    hipEvent_t a = nullptr, b = nullptr;
+    int rc;
+    if (timed && (rc = mark(c, &a))) return rc;
+    for (int s = 0; s < PT_NSPECIES; s++) {
+        if (sp >= 0 && s != sp) continue;
+        if ((rc = compact_species(c, s))) return rc;
+    }
+    if (timed) {
+        if ((rc = mark(c, &b))) return rc;
+        c->spans.push_back(Span{a, b, SPAN_CMP});
+    }
     return PT_OK;
 }
 
@@ -1507,21 +1758,8 @@ int pt_compact(pt_ctx* c, int sp) {
     if (!c) return PT_E_ARG;
     if (sp < -1 || sp >= PT_NSPECIES) return fail(c, PT_E_ARG, "bad species");
     HIPCHK(c, hipSetDevice(c->device));
-    int rc = read_counts(c);
+    int rc = compact_all(c, sp, c->profile);
     if (rc) return rc;
-    if (c->profile) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-    for (int s = 0; s < PT_NSPECIES; s++) {
-        if (sp >= 0 && s != sp) continue;
-        rc = compact_species(c, s);
-        if (rc) return rc;
-    }
-    if (c->profile) {
-        HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
-        HIPCHK(c, hipEventSynchronize(c->ev[4]));
-        float a = 0;
-        HIPCHK(c, hipEventElapsedTime(&a, c->ev[3], c->ev[4]));
-        c->cmp_ms += a;
-    }
     return read_counts(c);
 }
 
@@ -1535,12 +1773,12 @@ int pt_run(pt_ctx* c, double t0, double dt, int nsteps, int compact_every, int c
     if (check_every < 1) check_every = 1;
     double t = t0;
     for (int q = 0; q < nsteps; q++) {
-        rc = launch_step(c, t + dt);
+        rc = launch_step(c, false, t + dt);
         if (rc) return rc;
         t += dt;
         bool last = q == nsteps - 1;
         if (compact_every > 0 && (q + 1) % compact_every == 0) {
-            rc = pt_compact(c, -1);
+            rc = compact_all(c, -1, c->profile);
             if (rc) return rc;
         }
         if (last || (q + 1) % check_every == 0) {
@@ -1551,6 +1789,203 @@ int pt_run(pt_ctx* c, double t0, double dt, int nsteps, int compact_every, int c
         }
     }
     return PT_OK;
+}
+
+int pt_set_launch(pt_ctx* c, const pt_launch* l) {
+    if (!c) return PT_E_ARG;
+    if (!l || (l->block != 128 && l->block != 256 && l->block != 512) || l->grid_mult < 1 ||
+        l->grid_mult > 16 || l->compact_every < 1)
+        return fail(c, PT_E_ARG, "launch: block must be 128, 256 or 512, grid_mult in 1..16, compact_every >= 1");
+    c->launch = *l;
+    c->launch.pad = 0;
+    return PT_OK;
+}
+
+int pt_get_launch(pt_ctx* c, pt_launch* l) {
+    if (!c || !l) return PT_E_ARG;
+    *l = c->launch;
+    return PT_OK;
+}
+
+static bool same_launch(const pt_launch& a, const pt_launch& b) {
+    return a.block == b.block && a.grid_mult == b.grid_mult && a.compact_every == b.compact_every;
+}
+
+// capture `steps` clocked steps (+ one compaction of every species if `compact`)
+static int capture(pt_ctx* c, int which, int steps, bool compact) {
+    if (c->gexec[which]) {
+        HIPCHK(c, hipGraphExecDestroy(c->gexec[which]));
+        c->gexec[which] = nullptr;
+    }
+    int rc = ensure_residency(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int lrc = PT_OK;
+    for (int q = 0; q < steps && lrc == PT_OK; q++) lrc = launch_step(c, true, 0.0);
+    if (lrc == PT_OK && compact) lrc = compact_all(c, -1, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (lrc) {
+        if (g) hipGraphDestroy(g);
+        return lrc;
+    }
+    if (e != hipSuccess) return fail(c, PT_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->gexec[which], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(c, PT_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    c->gsteps[which] = steps;
+    c->glaunch[which] = c->launch;
+    c->glds[which] = c->lds_bytes;
+    return PT_OK;
+}
+
+static int replay(pt_ctx* c, int which) {
+    hipEvent_t a = nullptr, b = nullptr;
+    int rc;
+    if (c->profile && (rc = mark(c, &a))) return rc;
+    HIPCHK(c, hipGraphLaunch(c->gexec[which], c->stream));
+    if (c->profile) {
+        if ((rc = mark(c, &b))) return rc;
+        c->spans.push_back(Span{a, b, SPAN_GRAPH});
+    }
+    return PT_OK;
+}
+
+int pt_run_graph(pt_ctx* c, double t0, double dt, int nsteps, int check_every) {
+    if (!c) return PT_E_ARG;
+    if (nsteps < 0) return fail(c, PT_E_ARG, "nsteps < 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = require_tables(c);
+    if (rc) return rc;
+    rc = sync_params(c);
+    if (rc) return rc;
+    if (check_every < 1) check_every = 1;
+    const int ce = c->launch.compact_every;
+    const int full = nsteps / ce, rem = nsteps % ce;
+    if (full > 0 && (!c->gexec[0] || c->gsteps[0] != ce || !same_launch(c->glaunch[0], c->launch) ||
+                     c->glds[0] != c->lds_bytes)) {
+        if ((rc = capture(c, 0, ce, true))) return rc;
+    }
+    if (rem > 0 && (!c->gexec[1] || c->gsteps[1] != rem || !same_launch(c->glaunch[1], c->launch) ||
+                    c->glds[1] != c->lds_bytes)) {
+        if ((rc = capture(c, 1, rem, false))) return rc;
+    }
+    Clock ck{};
+    ck.t = t0;
+    ck.dt = dt;
+    ck.tfinal = t0;
+    ck.step = c->step;
+    HIPCHK(c, hipMemcpyAsync(c->dclock, &ck, sizeof(Clock), hipMemcpyHostToDevice, c->stream));
+    int done = 0, since = 0;
+    for (int k = 0; k < full; k++) {
+        if ((rc = replay(c, 0))) return rc;
+        done += ce;
+        since += ce;
+        if (since >= check_every && done < nsteps) {
+            since = 0;
+            if ((rc = check_flags(c, nullptr))) {
+                c->step += (uint32_t)done;
+                return rc;
+            }
+        }
+    }
+    if (rem > 0 && (rc = replay(c, 1))) return rc;
+    c->step += (uint32_t)nsteps;
+    if ((rc = read_counts(c))) return rc;
+    return check_flags(c, nullptr);
+}
+
+// particle-steps done so far (device work counter)
+static int psteps_now(pt_ctx* c, ull* v) {
+    ull w[NWORK];
+    HIPCHK(c, hipMemcpyAsync(w, c->dwork, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *v = w[W_PSTEPS];
+    return PT_OK;
+}
+
+int pt_autotune(pt_ctx* c, double t0, double dt, int trial_steps, pt_tune_trial* trials, int max_trials,
+                int* ntrials, pt_launch* best, double* t_end) {
+    if (!c) return PT_E_ARG;
+    if (trial_steps < 1) return fail(c, PT_E_ARG, "trial_steps must be >= 1");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = require_tables(c);
+    if (rc) return rc;
+    if ((rc = sync_params(c))) return rc;
+    const bool prof = c->profile;
+    c->profile = false;
+    double t = t0;
+    int nt = 0;
+    hipEvent_t a = c->ev[0], b = c->ev[1];
+    // one timed window: `steps` steps from t with compaction every `ce` (0: none)
+    auto window = [&](int steps, int ce, double* rate) -> int {
+        int r2;
+        if ((r2 = compact_all(c, -1, false))) return r2;
+        if ((r2 = read_counts(c))) return r2;
+        ull p0, p1;
+        if ((r2 = psteps_now(c, &p0))) return r2;
+        HIPCHK(c, hipEventRecord(a, c->stream));
+        for (int q = 0; q < steps; q++) {
+            if ((r2 = launch_step(c, false, t + dt))) return r2;
+            t += dt;
+            if (ce > 0 && (q + 1) % ce == 0 && (r2 = compact_all(c, -1, false))) return r2;
+        }
+        HIPCHK(c, hipEventRecord(b, c->stream));
+        HIPCHK(c, hipEventSynchronize(b));
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, a, b));
+        if ((r2 = psteps_now(c, &p1))) return r2;
+        if ((r2 = check_flags(c, nullptr))) return r2;
+        *rate = ms > 0 ? (double)(p1 - p0) / (ms * 1e-3) : 0.0;
+        return PT_OK;
+    };
+    pt_launch keep = c->launch, top = c->launch;
+    double top_rate = -1.0;
+    const int blocks[3] = {128, 256, 512}, mults[3] = {1, 2, 4};
+    for (int bi = 0; bi < 3 && rc == PT_OK; bi++)
+        for (int mi = 0; mi < 3 && rc == PT_OK; mi++) {
+            c->launch.block = blocks[bi];
+            c->launch.grid_mult = mults[mi];
+            double r = 0;
+            rc = window(trial_steps, 0, &r);
+            if (rc) break;
+            if (trials && nt < max_trials) trials[nt] = pt_tune_trial{c->launch, r};
+            nt++;
+            if (r > top_rate) {
+                top_rate = r;
+                top = c->launch;
+            }
+        }
+    if (rc == PT_OK) {
+        c->launch = top;
+        const int cad[5] = {1, 2, 4, 8, 16};
+        double best_rate = -1.0;
+        for (int k = 0; k < 5 && rc == PT_OK; k++) {
+            double r = 0;
+            rc = window(16, cad[k], &r);
+            if (rc) break;
+            pt_launch l = top;
+            l.compact_every = cad[k];
+            if (trials && nt < max_trials) trials[nt] = pt_tune_trial{l, r};
+            nt++;
+            if (r > best_rate) {
+                best_rate = r;
+                top.compact_every = cad[k];
+            }
+        }
+    }
+    c->profile = prof;
+    if (rc) {
+        c->launch = keep;
+        if (t_end) *t_end = t;
+        if (ntrials) *ntrials = nt;
+        return rc;
+    }
+    c->launch = top;
+    if (best) *best = top;
+    if (t_end) *t_end = t;
+    if (ntrials) *ntrials = nt;
+    return read_counts(c);
 }
 
 int pt_stats(pt_ctx* c, int sp, pt_pop_stats* out) {
@@ -1583,6 +2018,28 @@ int pt_stats(pt_ctx* c, int sp, pt_pop_stats* out) {
                  out->centroid[2] * out->centroid[2];
     out->spread = a[1] > 0 ? std::sqrt(std::fabs(a[6] / a[1] - xm2)) : 0.0;
     out->max_energy = a[7];
+    return PT_OK;
+}
+
+int pt_swarm_stats(pt_ctx* c, int sp, int nswarm, double* out) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES || !out || nswarm < 1 || nswarm > c->cfg.nswarm)
+        return fail(c, PT_E_ARG, "bad arguments (nswarm must be in 1..%d)", c->cfg.nswarm);
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = sync_params(c);
+    if (rc) return rc;
+    if (c->stat_cap < nswarm) {
+        if ((rc = dalloc(c, (void**)&c->swarm_part, sizeof(double) * 4 * (size_t)c->cfg.nswarm))) return rc;
+        c->stat_cap = c->cfg.nswarm;
+    }
+    rc = read_counts(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemsetAsync(c->swarm_part, 0, sizeof(double) * 4 * (size_t)nswarm, c->stream));
+    int blocks = (int)std::min<ull>((ull)c->grid, std::max<ull>(1, (c->hn[sp] + 255) / 256));
+    hipLaunchKernelGGL(k_swarm_stats, dim3(blocks), dim3(256), 0, c->stream, c->dp, sp, nswarm, c->swarm_part);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(out, c->swarm_part, sizeof(double) * 4 * (size_t)nswarm, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return PT_OK;
 }
 
@@ -1635,10 +2092,11 @@ int pt_work_read(pt_ctx* c, pt_work* out, int reset) {
     out->tail_ms = c->tail_ms;
     out->compact_ms = c->cmp_ms;
     out->main_secondaries = w[W_MAIN_SEC];
+    out->graph_ms = c->graph_ms;
     if (reset) {
         HIPCHK(c, hipMemsetAsync(c->dwork, 0, sizeof(w), c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        c->adv_ms = c->tail_ms = c->cmp_ms = 0.0;
+        c->adv_ms = c->tail_ms = c->cmp_ms = c->graph_ms = 0.0;
         c->adv_launches = 0;
     }
     return PT_OK;

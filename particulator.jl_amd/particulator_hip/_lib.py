@@ -71,7 +71,17 @@ class Work(C.Structure):
                 ("moved", C.c_uint64), ("events", C.c_uint64), ("steps", C.c_uint64),
                 ("tail_particles", C.c_uint64), ("advance_ms", C.c_double),
                 ("advance_launches", C.c_uint64), ("tail_ms", C.c_double),
-                ("compact_ms", C.c_double), ("main_secondaries", C.c_uint64)]
+                ("compact_ms", C.c_double), ("main_secondaries", C.c_uint64),
+                ("graph_ms", C.c_double)]
+
+
+class Launch(C.Structure):
+    _fields_ = [("block", C.c_int32), ("grid_mult", C.c_int32), ("compact_every", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class TuneTrial(C.Structure):
+    _fields_ = [("launch", Launch), ("rate", C.c_double)]
 
 
 DP = C.POINTER(C.c_double)
@@ -102,6 +112,12 @@ SIGNATURES = {
     "pt_run": (C.c_int, [VP, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]),
     "pt_compact": (C.c_int, [VP, C.c_int]),
     "pt_stats": (C.c_int, [VP, C.c_int, C.POINTER(PopStats)]),
+    "pt_set_launch": (C.c_int, [VP, C.POINTER(Launch)]),
+    "pt_get_launch": (C.c_int, [VP, C.POINTER(Launch)]),
+    "pt_run_graph": (C.c_int, [VP, C.c_double, C.c_double, C.c_int, C.c_int]),
+    "pt_autotune": (C.c_int, [VP, C.c_double, C.c_double, C.c_int, C.POINTER(TuneTrial), C.c_int,
+                              C.POINTER(C.c_int), C.POINTER(Launch), DP]),
+    "pt_swarm_stats": (C.c_int, [VP, C.c_int, C.c_int, DP]),
     "pt_collision_counts": (C.c_int, [VP, C.c_int, C.POINTER(C.c_int64), C.c_int, C.c_int]),
     "pt_tally_read": (C.c_int, [VP, C.c_int, DP, C.c_int, DP, DP]),
     "pt_work_read": (C.c_int, [VP, C.POINTER(Work), C.c_int]),

@@ -7,6 +7,8 @@ so argon ionisation is Moller(18, tcut) with tcut equal to the energy cut; argon
 part in every atomic process (elastic, bremsstrahlung, photoelectric, pair production,
 Compton, Bhabha, annihilation).
 """
+import numpy as np
+
 from . import constants as co
 from .physics import (BetheHeitler, Bhaba, Compton, Moller, ORBITALS, PhotoElectric,
                       PositronAnihilation, RelativisticCoulomb, SeltzerBerger)
@@ -68,6 +70,17 @@ def photon_processes(comp):
     procs += [(c, BetheHeitler(Z)) for Z, c in atoms]
     procs += [(c, Compton(Z)) for Z, c in atoms]
     return procs
+
+
+def swarm_tables(comp, efields, dt, cut, safety=1.1):
+    """Tables for independent swarms in fields efields[i] (V/m): one set of rate fits, and
+    per-swarm rate bounds of electrons and positrons (Fdt_i = e E_i dt, scripts/swarm.jl:47);
+    the photon bound does not depend on the field."""
+    te, tg, tp = build_tables(comp, efields[0], dt, cut, safety)
+    Fdts = [co.elementary_charge * e * dt for e in efields]
+    te = te.with_ratebounds(np.stack([te.ratebound_for(F, safety) for F in Fdts]))
+    tp = tp.with_ratebounds(np.stack([tp.ratebound_for(F, safety) for F in Fdts]))
+    return te, tg, tp
 
 
 def build_tables(comp, efield, dt, cut, safety=1.1):

