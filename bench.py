@@ -86,6 +86,11 @@ def main():
     ap.add_argument("--workload", default="avalanche", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=None, help="override the initial particle count")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--rounds", type=int, default=None, help="collision rounds per step (0: history mode)")
+    ap.add_argument("--block", type=int, default=None, help="threads per workgroup of the step kernels")
+    ap.add_argument("--grid-mult", type=int, default=None)
+    ap.add_argument("--defer-mask", type=lambda v: int(v, 0), default=None,
+                    help="sampler kinds queued in collision rounds (bit = pt_proc_kind)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=400_000)
     ap.add_argument("--cpu-steps", type=int, default=10)
@@ -125,7 +130,14 @@ def main():
     mp.init_particles(P.PT_ELECTRON, n, wl["e1"] * co.eV, (wl["e2"] or wl["e1"]) * co.eV,
                       loguniform=wl["loguniform"])
     mp.init()
-    log("setup %.1f s" % (time.perf_counter() - t_setup))
+    launch = mp.get_launch()
+    for k, v in (("rounds", args.rounds), ("block", args.block), ("grid_mult", args.grid_mult),
+                 ("defer_mask", args.defer_mask)):
+        if v is not None:
+            launch[k] = v
+    launch["compact_every"] = wl["compact_every"]
+    mp.set_launch(**launch)
+    log("setup %.1f s, launch %s" % (time.perf_counter() - t_setup, launch))
     dt = wl["dt"]
     t = mp.run_steps(0.0, dt, args.warmup, compact_every=wl["compact_every"], check_every=args.warmup or 1)
     mp.work(reset=True)

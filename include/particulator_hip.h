@@ -219,7 +219,13 @@ typedef struct {
     int32_t block;          /* threads per workgroup of the advance kernel: 128, 256 or 512 */
     int32_t grid_mult;      /* workgroups launched = resident workgroups x grid_mult (1..16) */
     int32_t compact_every;  /* steps between compactions in pt_run_graph (>= 1) */
-    int32_t pad;
+    int32_t rounds;         /* collision rounds per step (0..8): the push/select kernels queue
+                               real collisions of the kinds in defer_mask per sampler kind
+                               and sampling kernels apply them, `rounds` times, before a
+                               pass that samples every kind in place finishes the step;
+                               0: every collision is sampled in place */
+    uint32_t defer_mask;    /* bit k set: collisions of pt_proc_kind k go through the
+                               sampling kernels in collision rounds */
 } pt_launch;
 /* No reference counterpart (the reference's @batch threads split the loop statically,
  * src/mixed_population.jl:61). */
@@ -237,9 +243,9 @@ typedef struct {
     double rate;            /* particle-steps/s measured over the trial */
 } pt_tune_trial;
 /* Autotuner (no reference counterpart): times trial_steps steps of the live run for each
- * advance-kernel shape (block 128/256/512 x grid_mult 1/2/4), then 16 steps for each
- * compaction cadence 1/2/4/8/16 with the fastest shape, and sets the fastest as the
- * context's launch.  The trial steps advance the run from t0 by dt; *t_end is the time
+ * number of collision rounds (0/2/3/4), then for each kernel shape (block 128/256/512 x
+ * grid_mult 1/2/4) with the fastest rounds, then 16 steps for each compaction cadence
+ * 1/2/4/8/16 with the fastest shape, and sets the fastest as the context's launch.  The trial steps advance the run from t0 by dt; *t_end is the time
  * reached.  trials[] receives up to max_trials measurements. */
 int pt_autotune(pt_ctx* ctx, double t0, double dt, int trial_steps, pt_tune_trial* trials,
                 int max_trials, int* ntrials, pt_launch* best, double* t_end);

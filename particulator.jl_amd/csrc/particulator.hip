@@ -39,6 +39,9 @@ struct SpeciesDev {
     uint8_t* active;
     uint64_t* uid;
     int32_t* swarm;
+    double* trem;     // time left in the step of a particle waiting in a queue
+    uint32_t* draw;   // its position in its random stream
+    uint8_t* evp;     // the process selected for its pending collision
     ull* n;
     ull cap;
     int tracked;
@@ -58,9 +61,27 @@ constexpr int NCOUNT = MAX_PROCS + 1;
 // advance the time.
 struct Clock {
     double t, dt, tfinal;
-    uint32_t step, pad;
-    ull cursor;  // work items claimed in the main pass
+    uint32_t step, qpar;  // qpar: which work queue of each species is the input
+    ull cursor;           // work items claimed by the push/select kernel
+    ull ecur;             // queued collisions claimed by the sampling kernel
+    ull nmark[PT_NSPECIES];  // population counts where the last secondary pass starts
 };
+
+// Work queues of the event rounds: per species two ping-pong queues of particles to
+// advance and one queue of pending collisions per sampler kind present in its table.
+// Entries are slot indices; bit 31 marks a particle born in this step.
+constexpr int NKQ = 6;
+constexpr uint32_t CHILD_BIT = 0x80000000u;
+struct Queues {
+    uint32_t* work[PT_NSPECIES][2];
+    uint32_t* ev[PT_NSPECIES][NKQ];
+    ull* cnt;  // [3][2] work counts, then [3][NKQ] collision counts
+    int8_t kq[PT_NSPECIES][16];  // sampler kind -> collision queue (-1: none)
+    int nkq[PT_NSPECIES];
+};
+__host__ __device__ constexpr int QWK(int s, int par) { return 2 * s + par; }
+__host__ __device__ constexpr int QEV(int s, int k) { return 2 * PT_NSPECIES + NKQ * s + k; }
+constexpr int NQCNT = 2 * PT_NSPECIES + NKQ * PT_NSPECIES;
 
 struct Params {
     SpeciesDev sp[PT_NSPECIES];
@@ -79,6 +100,8 @@ struct Params {
     double tally_lo, tally_hi;
     ull* range;  // [3][2]
     Clock* clock;
+    Queues q;
+    uint32_t defer_mask;  // sampler kinds (bit = pt_proc_kind) queued in collision rounds
     int lds_rate_off[PT_NSPECIES];
     int lds_rb_off[PT_NSPECIES];
     int lds_doubles;
@@ -93,6 +116,9 @@ struct SpLite {
     uint8_t* active;
     uint64_t* uid;
     int32_t* swarm;
+    double* trem;
+    uint32_t* draw;
+    uint8_t* evp;
     ull* n;
     ull cap;
     const double* rate;  // LDS
@@ -126,6 +152,9 @@ __device__ __forceinline__ void stage(const Params& P, double* lds, Shared& sh) 
             L.active = S.active;
             L.uid = S.uid;
             L.swarm = S.swarm;
+            L.trem = S.trem;
+            L.draw = S.draw;
+            L.evp = S.evp;
             L.n = S.n;
             L.cap = S.cap;
             L.tracked = S.tracked;
@@ -180,7 +209,7 @@ constexpr int NCHILD = 2;
 
 __device__ __forceinline__ ull append(const Params& P, Shared& sh, int sp2, const V3& x,
                                       const V3& p, double w, double t, double s, uint64_t uid,
-                                      int swarm, bool main_pass) {
+                                      int swarm, bool main_pass, double tfinal) {
     const SpLite& L = sh.sp[sp2];
     double E = kinenergy(sp2, p);
     if (E <= L.cut) return NO_SLOT;
@@ -222,6 +251,8 @@ __device__ __forceinline__ ull append(const Params& P, Shared& sh, int sp2, cons
     L.uid[slot] = uid;
     L.swarm[slot] = swarm;
     L.active[slot] = 1;
+    L.trem[slot] = tfinal - t;
+    L.draw[slot] = 0;
     atomicAdd(&sh.bc.work[W_SECONDARIES], 1ull);
     if (main_pass) atomicAdd(&sh.bc.work[W_MAIN_SEC], 1ull);
     return slot;
@@ -233,17 +264,26 @@ __device__ __forceinline__ ull append(const Params& P, Shared& sh, int sp2, cons
 // event loop: each iteration advances a lane's particle by one push (and at most one
 // collision); a lane whose particle has reached tfinal takes its next particle at once, so
 // lanes of a wave stay busy whatever their particles' collision counts.  r is recomputed
-// when a particle is loaded (advance_init!) and droplow!'s cut is applied when it is done.
+// when a particle enters the step (advance_init!) and droplow!'s cut is applied when it
+// is done.
 //
-// Work items q in [0, total) index the concatenation of the three species' ranges
-// [rb[s], re[s]).  A wave claims chunks of CHUNK consecutive items from `cursor` (a device
-// counter for the main pass, an LDS counter for the tail passes) and hands them out in
-// order to its free lanes (ballot rank), so the particles a wave holds at any moment lie
-// in a short window of consecutive slots: refills read whole cache lines and write-backs
-// of neighbouring lanes meet in L2.  The secondaries a lane creates are advanced by that
+// Work items q index the concatenation of the three species' item lists: slot ranges
+// [rb[s], re[s]) (SRC_RANGE) or the entries of the species' input work queues
+// (SRC_QUEUE).  A wave claims chunks of CHUNK consecutive items from `cursor` and hands
+// them out in order to its free lanes (ballot rank), so the particles a wave holds lie in
+// a short window of consecutive slots: refills read whole cache lines and neighbouring
+// write-backs meet in L2.
+//
+// Collisions are sampled in the lane; the secondaries a lane creates are advanced by that
 // lane right after their parent (up to NCHILD pending per lane; the rest are left to the
 // tail passes, which skip every particle already at tfinal).
+// EVENT = true (collision rounds): a selected real collision of a sampler kind in
+// P.defer_mask is not sampled here; the particle is written back with its time left and
+// stream position and queued for that sampler kind (wave-aggregated appends), and the
+// lane moves on.  The sampling kernel then runs each queued kind with whole waves of the
+// same kind (no divergence between samplers; the learned samplers need it).
 constexpr int CHUNK = 256;
+enum { SRC_RANGE = 0, SRC_QUEUE = 1 };
 
 struct Loaded {
     V3 x, p;
@@ -267,8 +307,28 @@ __device__ __forceinline__ Loaded load_slot(const SpLite& L, ull idx) {
     return a;
 }
 
-__device__ __forceinline__ void process_range(const Params& P, Shared& sh, const ull* rb, const ull* re,
-                                              ull* cursor, double tfinal, uint32_t step, bool main_pass) {
+// Append `val` to the queue (arr, cnt) each lane names (cnt == nullptr: none).  Lanes
+// naming the same queue share one atomic and write consecutive entries.  Must be called
+// by the whole wave.
+__device__ __forceinline__ void wave_push(ull* cnt, uint32_t* arr, uint32_t val, int lane, uint64_t ltmask) {
+    uint64_t pend = __ballot(cnt != nullptr);
+    while (pend) {
+        int leader = __ffsll((unsigned long long)pend) - 1;
+        ull key = __shfl((ull)(uintptr_t)cnt, leader);
+        bool mine = cnt != nullptr && (ull)(uintptr_t)cnt == key;
+        uint64_t m = __ballot(mine);
+        ull base = 0;
+        if (lane == leader) base = atomicAdd((ull*)(uintptr_t)key, (ull)__popcll(m));
+        base = __shfl(base, leader);
+        if (mine) arr[base + (ull)__popcll(m & ltmask)] = val;
+        pend &= ~m;
+    }
+}
+
+template <int SRC, bool EVENT>
+__device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const ull* rb, const ull* re,
+                                            ull* cursor, double tfinal, uint32_t step, bool main_pass,
+                                            int par) {
     const ull n0 = re[0] - rb[0], n1 = re[1] - rb[1], n2 = re[2] - rb[2];
     const ull total = n0 + n1 + n2;
     const int lane = threadIdx.x & 63;
@@ -292,31 +352,33 @@ __device__ __forceinline__ void process_range(const Params& P, Shared& sh, const
     ull chidx[NCHILD];
     for (;;) {
         // 1. pending secondaries of this lane
-        while (!have && nch > 0) {
-            nch--;
-            int csp = nch == 0 ? chsp[0] : chsp[1];
-            ull cidx = nch == 0 ? chidx[0] : chidx[1];
-            Loaded a = load_slot(sh.sp[csp], cidx);
-            double cr = tfinal - a.t;
-            if (a.act && cr > EPS64) {
-                sp = csp;
-                idx = cidx;
-                x = a.x;
-                p = a.p;
-                w = a.w;
-                t = a.t;
-                s = a.s;
-                uid = a.uid;
-                swarm = (a.swarm < 0 || a.swarm >= P.nswarm) ? 0 : a.swarm;
-                trem = cr;
-                oob = false;
-                r = setr(sh.sp[sp], kinenergy(sp, p), swarm, oob);
-                g.init(P.seed, uid, step);
-                iters = 0;
-                nev = 0;
-                active = true;
-                have = true;
-                child = true;
+        {
+            while (!have && nch > 0) {
+                nch--;
+                int csp = nch == 0 ? chsp[0] : chsp[1];
+                ull cidx = nch == 0 ? chidx[0] : chidx[1];
+                Loaded a = load_slot(sh.sp[csp], cidx);
+                double cr = tfinal - a.t;
+                if (a.act && cr > EPS64) {
+                    sp = csp;
+                    idx = cidx;
+                    x = a.x;
+                    p = a.p;
+                    w = a.w;
+                    t = a.t;
+                    s = a.s;
+                    uid = a.uid;
+                    swarm = (a.swarm < 0 || a.swarm >= P.nswarm) ? 0 : a.swarm;
+                    trem = cr;
+                    oob = false;
+                    r = setr(sh.sp[sp], kinenergy(sp, p), swarm, oob);
+                    g.init(P.seed, uid, step);
+                    iters = 0;
+                    nev = 0;
+                    active = true;
+                    have = true;
+                    child = true;
+                }
             }
         }
         // 2. free lanes take the next items of the wave's chunk, in lane order
@@ -351,9 +413,25 @@ __device__ __forceinline__ void process_range(const Params& P, Shared& sh, const
                     qs = 2;
                     qi = rb[2] + (q - n0 - n1);
                 }
-                Loaded a = load_slot(sh.sp[qs], qi);
-                double cr = tfinal - a.t;
-                if (a.act && cr > EPS64) {
+                bool qchild = !main_pass;
+                if constexpr (SRC == SRC_QUEUE) {
+                    uint32_t e = P.q.work[qs][par][qi];
+                    qchild = (e & CHILD_BIT) != 0;
+                    qi = e & ~CHILD_BIT;
+                }
+                const SpLite& Lq = sh.sp[qs];
+                Loaded a = load_slot(Lq, qi);
+                double cr;
+                uint32_t d0 = 0;
+                if constexpr (SRC == SRC_QUEUE) {
+                    cr = Lq.trem[qi];
+                    d0 = Lq.draw[qi];
+                } else {
+                    cr = tfinal - a.t;
+                }
+                // queued particles are active and always enter (one with no time left is
+                // only finished); range items enter when active with time left
+                if (SRC == SRC_QUEUE || (a.act && cr > EPS64)) {
                     sp = qs;
                     idx = qi;
                     x = a.x;
@@ -365,128 +443,166 @@ __device__ __forceinline__ void process_range(const Params& P, Shared& sh, const
                     swarm = (a.swarm < 0 || a.swarm >= P.nswarm) ? 0 : a.swarm;
                     trem = cr;
                     oob = false;
-                    r = setr(sh.sp[sp], kinenergy(sp, p), swarm, oob);
+                    if constexpr (SRC == SRC_QUEUE) {
+                        r = Lq.f[PT_R][qi];  // set by the sampler (setr! of apply!) or at birth
+                    } else {
+                        r = setr(Lq, kinenergy(qs, p), swarm, oob);
+                    }
                     g.init(P.seed, uid, step);
+                    g.draw = d0;
                     iters = 0;
                     nev = 0;
                     active = true;
                     have = true;
-                    child = !main_pass;
+                    child = qchild;
                 }
             }
             need = __ballot(!have);
         }
         // 3. the wave leaves when its items are claimed and no lane holds a particle
         if (drained && __ballot(have) == 0) break;
-        if (!have) continue;
         // 4. one push (and at most one collision) of this lane's particle
-        const SpLite& L = sh.sp[sp];
-        if (++iters > ITER_LIMIT || g.exhausted()) {
-            atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
-            trem = 0.0;
-        } else {
-            double tnext = s / r;
-            double dt;
-            bool coll;
-            if (trem > tnext) {
-                dt = tnext;
-                coll = true;
-            } else {
-                dt = trem;
-                coll = false;
-                s -= dt * r;
-            }
-            push(sp, P.fields[swarm], x, p, t, dt);
-            if (coll && r != 0.0) {
-                double E = kinenergy(sp, p);
-                if (E >= L.cut) {
-                    nev++;
-                    double t1, t2;
-                    bool o;
-                    int iv = precheb(E, L.k, L.xmax, t1, t2, o);
-                    oob |= o;
-                    double xi = g.next() * r;
-                    const double* rt = L.rate + (size_t)iv * L.nprocs * 3;
-                    int sel = -1;
-                    for (int j = 0; j < L.nprocs; j++) {
-                        double nu = cheb3(rt + 3 * j, t1, t2);
-                        if (nu > xi) {
-                            sel = j;
-                            break;
-                        }
-                        xi -= nu;
-                    }
-                    if (sel < 0) {
-                        if (!(xi >= 0.0)) atomicAdd(&sh.bc.flags[FLAG_RATEBOUND], 1ull);
-                        atomicAdd(&sh.bc.counts[sp * NCOUNT], 1ull);
-                        r = setr(L, E, swarm, oob);
-                        s = nextcoll(g);
+        ull* qc = nullptr;
+        uint32_t* qa = nullptr;
+        uint32_t qv = 0;
+        if (have) {
+            const SpLite& L = sh.sp[sp];
+            int qsel = -1;
+            if (trem > EPS64 && active) {
+                if (++iters > ITER_LIMIT || g.exhausted()) {
+                    atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
+                    trem = 0.0;
+                } else {
+                    double tnext = s / r;
+                    double dt;
+                    bool coll;
+                    if (trem > tnext) {
+                        dt = tnext;
+                        coll = true;
                     } else {
-                        atomicAdd(&sh.bc.counts[sp * NCOUNT + 1 + sel], 1ull);
-                        double pn = norm3(p);
-                        Ev e;
-                        sample_event(sh.procs[sp][sel], sp, p, pn, E, P.sb, P.pe, g, e);
-                        if (e.err) atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
-                        V3 m0, m1;
-                        double s0, s1;
-                        finish_event(p, pn, e, g, m0, m1, s0, s1);
-                        uint32_t dend = g.draw;
-                        if (e.keep) {
-                            p = m0;
-                            s = s0;
-                            r = setr(L, kinenergy(sp, p), swarm, oob);
-                        } else {
-                            active = false;
-                            ull sl = append(P, sh, e.sp0, x, m0, w, t, s0,
-                                            child_uid(P.seed, uid, step, dend, 0), swarm, main_pass);
-                            if (sl != NO_SLOT && nch < NCHILD) {
-                                if (nch == 0) {
-                                    chsp[0] = e.sp0;
-                                    chidx[0] = sl;
-                                } else {
-                                    chsp[1] = e.sp0;
-                                    chidx[1] = sl;
+                        dt = trem;
+                        coll = false;
+                        s -= dt * r;
+                    }
+                    push(sp, P.fields[swarm], x, p, t, dt);
+                    if (coll && r != 0.0) {
+                        double E = kinenergy(sp, p);
+                        if (E >= L.cut) {
+                            nev++;
+                            double t1, t2;
+                            bool o;
+                            int iv = precheb(E, L.k, L.xmax, t1, t2, o);
+                            oob |= o;
+                            double xi = g.next() * r;
+                            const double* rt = L.rate + (size_t)iv * L.nprocs * 3;
+                            int sel = -1;
+                            for (int j = 0; j < L.nprocs; j++) {
+                                double nu = cheb3(rt + 3 * j, t1, t2);
+                                if (nu > xi) {
+                                    sel = j;
+                                    break;
                                 }
-                                nch++;
+                                xi -= nu;
                             }
-                        }
-                        if (e.ne == 2) {
-                            ull sl = append(P, sh, e.sp1, x, m1, w, t, s1,
-                                            child_uid(P.seed, uid, step, dend, e.keep ? 0 : 1), swarm,
-                                            main_pass);
-                            if (sl != NO_SLOT && nch < NCHILD) {
-                                if (nch == 0) {
-                                    chsp[0] = e.sp1;
-                                    chidx[0] = sl;
+                            if (sel < 0) {
+                                if (!(xi >= 0.0)) atomicAdd(&sh.bc.flags[FLAG_RATEBOUND], 1ull);
+                                atomicAdd(&sh.bc.counts[sp * NCOUNT], 1ull);
+                                r = setr(L, E, swarm, oob);
+                                s = nextcoll(g);
+                            } else {
+                                atomicAdd(&sh.bc.counts[sp * NCOUNT + 1 + sel], 1ull);
+                                if (EVENT && ((P.defer_mask >> sh.procs[sp][sel].kind) & 1u)) {
+                                    qsel = sel;
                                 } else {
-                                    chsp[1] = e.sp1;
-                                    chidx[1] = sl;
+                                    double pn = norm3(p);
+                                    Ev e;
+                                    sample_event(sh.procs[sp][sel], sp, p, pn, E, P.sb, P.pe, g, e);
+                                    if (e.err) atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
+                                    V3 m0, m1;
+                                    double s0, s1;
+                                    finish_event(p, pn, e, g, m0, m1, s0, s1);
+                                    uint32_t dend = g.draw;
+                                    if (e.keep) {
+                                        p = m0;
+                                        s = s0;
+                                        r = setr(L, kinenergy(sp, p), swarm, oob);
+                                    } else {
+                                        active = false;
+                                        ull sl = append(P, sh, e.sp0, x, m0, w, t, s0,
+                                                        child_uid(P.seed, uid, step, dend, 0), swarm, main_pass,
+                                                        tfinal);
+                                        if (sl != NO_SLOT && nch < NCHILD) {
+                                            if (nch == 0) {
+                                                chsp[0] = e.sp0;
+                                                chidx[0] = sl;
+                                            } else {
+                                                chsp[1] = e.sp0;
+                                                chidx[1] = sl;
+                                            }
+                                            nch++;
+                                        }
+                                    }
+                                    if (e.ne == 2) {
+                                        ull sl = append(P, sh, e.sp1, x, m1, w, t, s1,
+                                                        child_uid(P.seed, uid, step, dend, e.keep ? 0 : 1), swarm,
+                                                        main_pass, tfinal);
+                                        if (sl != NO_SLOT && nch < NCHILD) {
+                                            if (nch == 0) {
+                                                chsp[0] = e.sp1;
+                                                chidx[0] = sl;
+                                            } else {
+                                                chsp[1] = e.sp1;
+                                                chidx[1] = sl;
+                                            }
+                                            nch++;
+                                        }
+                                    }
                                 }
-                                nch++;
                             }
                         }
                     }
+                    trem -= dt;
                 }
             }
-            trem -= dt;
+            if (EVENT && qsel >= 0) {
+                // queued for its sampler: the state after the push, the time left and the
+                // stream position (r and s are replaced when the collision is applied)
+                L.f[PT_X][idx] = x.x;
+                L.f[PT_Y][idx] = x.y;
+                L.f[PT_Z][idx] = x.z;
+                L.f[PT_PX][idx] = p.x;
+                L.f[PT_PY][idx] = p.y;
+                L.f[PT_PZ][idx] = p.z;
+                L.f[PT_T][idx] = t;
+                L.trem[idx] = trem;
+                L.draw[idx] = g.draw;
+                L.evp[idx] = (uint8_t)qsel;
+                int k = P.q.kq[sp][sh.procs[sp][qsel].kind];
+                qc = P.q.cnt + QEV(sp, k);
+                qa = P.q.ev[sp][k];
+                qv = (uint32_t)idx | (child ? CHILD_BIT : 0u);
+                if (oob) atomicAdd(&sh.bc.flags[FLAG_RANGE], 1ull);
+                if (nev) atomicAdd(&sh.bc.work[W_EVENTS], (ull)nev);
+                have = false;
+            } else if (!(trem > EPS64) || !active) {
+                if (active && kinenergy(sp, p) < L.cut) active = false;  // droplow!
+                L.f[PT_X][idx] = x.x;
+                L.f[PT_Y][idx] = x.y;
+                L.f[PT_Z][idx] = x.z;
+                L.f[PT_PX][idx] = p.x;
+                L.f[PT_PY][idx] = p.y;
+                L.f[PT_PZ][idx] = p.z;
+                L.f[PT_T][idx] = t;
+                L.f[PT_S][idx] = s;
+                L.f[PT_R][idx] = r;
+                if (!active) L.active[idx] = 0;
+                if (oob) atomicAdd(&sh.bc.flags[FLAG_RANGE], 1ull);
+                if (nev) atomicAdd(&sh.bc.work[W_EVENTS], (ull)nev);
+                atomicAdd(&sh.bc.work[child ? W_TAIL : W_PSTEPS], 1ull);
+                have = false;
+            }
         }
-        if (!(trem > EPS64) || !active) {
-            if (active && kinenergy(sp, p) < L.cut) active = false;  // droplow!
-            L.f[PT_X][idx] = x.x;
-            L.f[PT_Y][idx] = x.y;
-            L.f[PT_Z][idx] = x.z;
-            L.f[PT_PX][idx] = p.x;
-            L.f[PT_PY][idx] = p.y;
-            L.f[PT_PZ][idx] = p.z;
-            L.f[PT_T][idx] = t;
-            L.f[PT_S][idx] = s;
-            L.f[PT_R][idx] = r;
-            if (!active) L.active[idx] = 0;
-            if (oob) atomicAdd(&sh.bc.flags[FLAG_RANGE], 1ull);
-            if (nev) atomicAdd(&sh.bc.work[W_EVENTS], (ull)nev);
-            atomicAdd(&sh.bc.work[child ? W_TAIL : W_PSTEPS], 1ull);
-            have = false;
-        }
+        if constexpr (EVENT) wave_push(qc, qa, qv, lane, ltmask);
     }
 }
 
@@ -494,14 +610,16 @@ __device__ __forceinline__ void process_range(const Params& P, Shared& sh, const
 __global__ void k_snapshot(const Params* __restrict__ Pp, int clocked, double tfinal, uint32_t step) {
     const Params& P = *Pp;
     int s = threadIdx.x;
+    Clock& C = *P.clock;
     if (s < PT_NSPECIES) {
         ull n = *P.sp[s].n;
         if (n > P.sp[s].cap) n = P.sp[s].cap;
         P.range[2 * s] = 0;
         P.range[2 * s + 1] = n;
+        C.nmark[s] = n;
     }
+    for (int q = threadIdx.x; q < NQCNT; q += blockDim.x) P.q.cnt[q] = 0;
     if (s == 0) {
-        Clock& C = *P.clock;
         if (clocked) {
             C.tfinal = C.t + C.dt;
         } else {
@@ -509,6 +627,8 @@ __global__ void k_snapshot(const Params* __restrict__ Pp, int clocked, double tf
             C.step = step;
         }
         C.cursor = 0;
+        C.ecur = 0;
+        C.qpar = 0;
     }
 }
 
@@ -520,24 +640,179 @@ __global__ void k_tick(const Params* __restrict__ Pp) {
     }
 }
 
-template <int BS>
+// after a sampling round: its output queue becomes the input of the next push/select
+// round; the consumed input and the collision queues are emptied.  mark: the next round
+// is the last (history mode), whose overflow secondaries the tail passes take from the
+// current counts on.
+__global__ void k_rotate(const Params* __restrict__ Pp, int mark) {
+    const Params& P = *Pp;
+    Clock& C = *P.clock;
+    __shared__ uint32_t par;
+    if (threadIdx.x == 0) par = C.qpar;
+    __syncthreads();
+    int s = threadIdx.x;
+    if (s < PT_NSPECIES) {
+        P.q.cnt[QWK(s, par)] = 0;
+        for (int k = 0; k < NKQ; k++) P.q.cnt[QEV(s, k)] = 0;
+        if (mark) {
+            ull n = __hip_atomic_load(P.sp[s].n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            C.nmark[s] = n > P.sp[s].cap ? P.sp[s].cap : n;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        C.qpar = par ^ 1u;
+        C.cursor = 0;
+        C.ecur = 0;
+    }
+}
+
+// push/select over the step's particles (first round) or the input work queues
+template <int BS, int SRC, bool EVENT>
 __global__ void __launch_bounds__(BS) k_advance(const Params* __restrict__ Pp) {
     extern __shared__ double lds[];
     __shared__ Shared sh;
+    __shared__ int skip;
     const Params& P = *Pp;
-    stage(P, lds, sh);
-    ull rb[PT_NSPECIES], re[PT_NSPECIES];
-    for (int s = 0; s < PT_NSPECIES; s++) {
-        rb[s] = P.range[2 * s];
-        re[s] = P.range[2 * s + 1];
-    }
     const Clock& C = *P.clock;
-    process_range(P, sh, rb, re, &P.clock->cursor, C.tfinal, C.step, true);
+    const int par = (int)C.qpar;
+    ull rb[PT_NSPECIES], re[PT_NSPECIES];
+    ull total = 0;
+    for (int s = 0; s < PT_NSPECIES; s++) {
+        if (SRC == SRC_RANGE) {
+            rb[s] = P.range[2 * s];
+            re[s] = P.range[2 * s + 1];
+        } else {
+            rb[s] = 0;
+            re[s] = P.q.cnt[QWK(s, par)];
+        }
+        total += re[s] - rb[s];
+    }
+    // workgroups that start after every item is claimed leave before staging the tables
+    if (threadIdx.x == 0)
+        skip = __hip_atomic_load(&P.clock->cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= total;
+    __syncthreads();
+    if (skip) return;
+    stage(P, lds, sh);
+    select_loop<SRC, EVENT>(P, sh, rb, re, &P.clock->cursor, C.tfinal, C.step, SRC == SRC_RANGE, par);
     flush_counters(P, &sh.bc);
 }
 
-// Secondaries born in this step: one workgroup repeats passes over [begin, n) until a pass
-// appends nothing (the `while n > 0` loop of advance!, src/mixed_population.jl:43-46).
+// Sampling round: each wave takes 64 queued collisions of one sampler kind (the queues are
+// laid out back to back, each padded to whole tiles), samples and applies them (collide /
+// apply!, src/collisions.jl:83-133), and queues the surviving primary and the new
+// particles for the next push/select round.
+template <int BS>
+__global__ void __launch_bounds__(BS) k_sample(const Params* __restrict__ Pp) {
+    extern __shared__ double lds[];
+    __shared__ Shared sh;
+    __shared__ ull seg[PT_NSPECIES * NKQ + 1];
+    __shared__ int skip;
+    const Params& P = *Pp;
+    const Clock& C = *P.clock;
+    const double tfinal = C.tfinal;
+    const uint32_t step = C.step;
+    const int par = (int)C.qpar, nxt = par ^ 1;
+    if (threadIdx.x == 0) {
+        ull acc = 0;
+        for (int q = 0; q < PT_NSPECIES * NKQ; q++) {
+            seg[q] = acc;
+            acc += (P.q.cnt[2 * PT_NSPECIES + q] + 63) & ~63ull;
+        }
+        seg[PT_NSPECIES * NKQ] = acc;
+        skip = __hip_atomic_load(&P.clock->ecur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= acc;
+    }
+    __syncthreads();
+    if (skip) return;
+    stage(P, lds, sh);
+    const int lane = threadIdx.x & 63;
+    const uint64_t ltmask = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const ull total = seg[PT_NSPECIES * NKQ];
+    for (;;) {
+        int fl = __ffsll((unsigned long long)__ballot(1)) - 1;
+        ull b = 0;
+        if (lane == fl) b = atomicAdd(&P.clock->ecur, 64ull);
+        b = __shfl(b, fl);
+        if (b >= total) break;
+        int qi = 0;
+        while (qi + 1 < PT_NSPECIES * NKQ && seg[qi + 1] <= b) qi++;
+        const int sp = qi / NKQ, kq = qi % NKQ;
+        const ull pos = b - seg[qi] + (ull)lane;
+        ull* c0 = nullptr;
+        ull* c1 = nullptr;
+        ull* c2 = nullptr;
+        uint32_t* a0 = nullptr;
+        uint32_t* a1 = nullptr;
+        uint32_t* a2 = nullptr;
+        uint32_t v0 = 0, v1 = 0, v2 = 0;
+        if (pos < P.q.cnt[QEV(sp, kq)]) {
+            const SpLite& L = sh.sp[sp];
+            uint32_t ent = P.q.ev[sp][kq][pos];
+            const bool child = (ent & CHILD_BIT) != 0;
+            const ull idx = ent & ~CHILD_BIT;
+            V3 x{L.f[PT_X][idx], L.f[PT_Y][idx], L.f[PT_Z][idx]};
+            V3 p{L.f[PT_PX][idx], L.f[PT_PY][idx], L.f[PT_PZ][idx]};
+            const double w = L.f[PT_W][idx], t = L.f[PT_T][idx];
+            const uint64_t uid = L.uid[idx];
+            int swarm = L.swarm[idx];
+            if (swarm < 0 || swarm >= P.nswarm) swarm = 0;
+            const int sel = L.evp[idx];
+            PhiloxStream g;
+            g.init(P.seed, uid, step);
+            g.draw = L.draw[idx];
+            double E = kinenergy(sp, p);
+            double pn = norm3(p);
+            Ev e;
+            sample_event(sh.procs[sp][sel], sp, p, pn, E, P.sb, P.pe, g, e);
+            if (e.err) atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
+            V3 m0, m1;
+            double s0, s1;
+            finish_event(p, pn, e, g, m0, m1, s0, s1);
+            const uint32_t dend = g.draw;
+            if (e.keep) {
+                bool oob = false;
+                double r = setr(L, kinenergy(sp, m0), swarm, oob);
+                if (oob) atomicAdd(&sh.bc.flags[FLAG_RANGE], 1ull);
+                L.f[PT_PX][idx] = m0.x;
+                L.f[PT_PY][idx] = m0.y;
+                L.f[PT_PZ][idx] = m0.z;
+                L.f[PT_S][idx] = s0;
+                L.f[PT_R][idx] = r;
+                L.draw[idx] = dend;
+                c0 = P.q.cnt + QWK(sp, nxt);
+                a0 = P.q.work[sp][nxt];
+                v0 = ent;
+            } else {
+                L.active[idx] = 0;
+                atomicAdd(&sh.bc.work[child ? W_TAIL : W_PSTEPS], 1ull);
+                ull sl = append(P, sh, e.sp0, x, m0, w, t, s0, child_uid(P.seed, uid, step, dend, 0), swarm,
+                                false, tfinal);
+                if (sl != NO_SLOT) {
+                    c1 = P.q.cnt + QWK(e.sp0, nxt);
+                    a1 = P.q.work[e.sp0][nxt];
+                    v1 = (uint32_t)sl | CHILD_BIT;
+                }
+            }
+            if (e.ne == 2) {
+                ull sl = append(P, sh, e.sp1, x, m1, w, t, s1, child_uid(P.seed, uid, step, dend, e.keep ? 0 : 1),
+                                swarm, false, tfinal);
+                if (sl != NO_SLOT) {
+                    c2 = P.q.cnt + QWK(e.sp1, nxt);
+                    a2 = P.q.work[e.sp1][nxt];
+                    v2 = (uint32_t)sl | CHILD_BIT;
+                }
+            }
+        }
+        wave_push(c0, a0, v0, lane, ltmask);
+        wave_push(c1, a1, v1, lane, ltmask);
+        wave_push(c2, a2, v2, lane, ltmask);
+    }
+    flush_counters(P, &sh.bc);
+}
+
+// Secondaries born in this step and not yet advanced: one workgroup repeats passes over
+// [begin, n) until a pass appends nothing (the `while n > 0` loop of advance!,
+// src/mixed_population.jl:43-46).
 __global__ void __launch_bounds__(512) k_tail(const Params* __restrict__ Pp, int maxpasses) {
     extern __shared__ double lds[];
     __shared__ Shared sh;
@@ -549,7 +824,7 @@ __global__ void __launch_bounds__(512) k_tail(const Params* __restrict__ Pp, int
     const double tfinal = P.clock->tfinal;
     const uint32_t step = P.clock->step;
     if (threadIdx.x == 0)
-        for (int s = 0; s < PT_NSPECIES; s++) beg[s] = P.range[2 * s + 1];
+        for (int s = 0; s < PT_NSPECIES; s++) beg[s] = P.clock->nmark[s];
     if (threadIdx.x == 0) done = 0;
     for (int pass = 0; pass < maxpasses; pass++) {
         __threadfence();
@@ -571,7 +846,7 @@ __global__ void __launch_bounds__(512) k_tail(const Params* __restrict__ Pp, int
             rb[s] = beg[s];
             re[s] = end[s];
         }
-        process_range(P, sh, rb, re, &cur, tfinal, step, false);
+        select_loop<SRC_RANGE, false>(P, sh, rb, re, &cur, tfinal, step, false, 0);
         __threadfence();
         __syncthreads();
         if (threadIdx.x == 0)
@@ -1085,8 +1360,9 @@ struct pt_ctx {
     int ncu = 0;          // compute units
     int lds_bytes = 0;
     Clock* dclock = nullptr;
-    pt_launch launch{256, 1, 8, 0};
-    int resident[3] = {0, 0, 0};  // resident advance workgroups per CU for block 128/256/512
+    pt_launch launch{256, 1, 8, 0, 1u << PT_PROC_SB};
+    int resident[5][3] = {};  // resident workgroups per CU: kernel variant x block 128/256/512
+    bool queues = false;      // work and collision queues allocated
     int resident_lds = -1;        // LDS size the residency was computed for
     int stat_cap = 0;             // swarms the per-swarm statistics buffer holds
     double* swarm_part = nullptr;
@@ -1165,6 +1441,25 @@ struct Scratch {
     }
 };
 
+// kernel variants: history pass over the step's slots, first collision round, later
+// collision rounds, history pass over the work queues, sampling round
+enum { KV_RANGE_FULL = 0, KV_RANGE_EVENT = 1, KV_QUEUE_EVENT = 2, KV_QUEUE_FULL = 3, KV_SAMPLE = 4, NKV = 5 };
+
+template <int BS>
+static const void* kernel_variant(int v) {
+    switch (v) {
+        case KV_RANGE_FULL: return (const void*)k_advance<BS, SRC_RANGE, false>;
+        case KV_RANGE_EVENT: return (const void*)k_advance<BS, SRC_RANGE, true>;
+        case KV_QUEUE_EVENT: return (const void*)k_advance<BS, SRC_QUEUE, true>;
+        case KV_QUEUE_FULL: return (const void*)k_advance<BS, SRC_QUEUE, false>;
+        default: return (const void*)k_sample<BS>;
+    }
+}
+
+static const void* kernel_of(int v, int bs) {
+    return bs == 128 ? kernel_variant<128>(v) : bs == 512 ? kernel_variant<512>(v) : kernel_variant<256>(v);
+}
+
 extern "C" {
 
 int pt_abi_version(void) { return PT_ABI_VERSION; }
@@ -1200,8 +1495,8 @@ int pt_create(const pt_config* cfg, pt_ctx** out) {
         return rc;
     }
     for (int s = 0; s < PT_NSPECIES; s++) {
-        if (c->cfg.capacity[s] >= (1ull << 32)) {
-            int rc = fail(c, PT_E_ARG, "capacity of species %d exceeds 2^32 - 1", s);
+        if (c->cfg.capacity[s] >= (1ull << 31)) {
+            int rc = fail(c, PT_E_ARG, "capacity of species %d exceeds 2^31 - 1", s);
             pt_destroy(c);
             return rc;
         }
@@ -1236,6 +1531,9 @@ int pt_create(const pt_config* cfg, pt_ctx** out) {
         TRY(dalloc(c, (void**)&S.active, cap));
         TRY(dalloc(c, (void**)&S.uid, sizeof(uint64_t) * cap));
         TRY(dalloc(c, (void**)&S.swarm, sizeof(int32_t) * cap));
+        TRY(dalloc(c, (void**)&S.trem, sizeof(double) * cap));
+        TRY(dalloc(c, (void**)&S.draw, sizeof(uint32_t) * cap));
+        TRY(dalloc(c, (void**)&S.evp, cap));
         HIPCHK(c, hipMemsetAsync(S.active, 0, cap ? cap : 1, c->stream));
     }
     HIPCHK(c, hipMemsetAsync(c->dn, 0, sizeof(ull) * PT_NSPECIES, c->stream));
@@ -1244,6 +1542,10 @@ int pt_create(const pt_config* cfg, pt_ctx** out) {
     TRY(dalloc(c, (void**)&c->dwork, sizeof(ull) * NWORK));
     TRY(dalloc(c, (void**)&c->drange, sizeof(ull) * 2 * PT_NSPECIES));
     TRY(dalloc(c, (void**)&c->dclock, sizeof(Clock)));
+    TRY(dalloc(c, (void**)&P.q.cnt, sizeof(ull) * NQCNT));
+    HIPCHK(c, hipMemsetAsync(P.q.cnt, 0, sizeof(ull) * NQCNT, c->stream));
+    for (int s = 0; s < PT_NSPECIES; s++)
+        for (int k = 0; k < 16; k++) P.q.kq[s][k] = -1;
     HIPCHK(c, hipMemsetAsync(c->dclock, 0, sizeof(Clock), c->stream));
     HIPCHK(c, hipMemsetAsync(c->dcounts, 0, sizeof(ull) * PT_NSPECIES * NCOUNT, c->stream));
     HIPCHK(c, hipMemsetAsync(c->dflags, 0, sizeof(ull) * NFLAGS, c->stream));
@@ -1269,6 +1571,7 @@ int pt_create(const pt_config* cfg, pt_ctx** out) {
     P.work = c->dwork;
     P.range = c->drange;
     P.clock = c->dclock;
+    P.defer_mask = c->launch.defer_mask;
     // compaction scratch
     c->cchunks = (maxcap + CH - 1) / CH + 1;
     TRY(dalloc(c, (void**)&c->ccnt, sizeof(unsigned) * c->cchunks));
@@ -1352,6 +1655,17 @@ int pt_upload_table(pt_ctx* c, int sp, const pt_table_desc* t) {
         if (s.kind == PT_PROC_BETHE_HEITLER && !(s.c[0] > 0))
             return fail(c, PT_E_ARG, "process %d: Bethe-Heitler needs c[0] = Z^(1/3)", j);
     }
+    // one collision queue per sampler kind present in the table
+    Queues& Q = c->hp.q;
+    for (int k = 0; k < 16; k++) Q.kq[sp][k] = -1;
+    Q.nkq[sp] = 0;
+    for (int j = 0; j < t->nprocs; j++) {
+        int kd = t->procs[j].kind;
+        if (Q.kq[sp][kd] >= 0) continue;
+        if (Q.nkq[sp] >= NKQ) return fail(c, PT_E_ARG, "more than %d sampler kinds in one table", NKQ);
+        Q.kq[sp][kd] = (int8_t)Q.nkq[sp]++;
+    }
+    c->queues = false;  // allocate the new kinds' queues before the next collision round
     c->dirty = true;
     int rc = sync_params(c);
     if (rc) return rc;
@@ -1628,46 +1942,79 @@ static int block_index(int bs) { return bs == 128 ? 0 : bs == 512 ? 2 : 1; }
 
 static int ensure_residency(pt_ctx* c) {
     if (c->resident_lds == c->lds_bytes) return PT_OK;
-    int nb = 0;
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_advance<128>, 128, c->lds_bytes));
-    c->resident[0] = std::max(nb, 1);
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_advance<256>, 256, c->lds_bytes));
-    c->resident[1] = std::max(nb, 1);
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_advance<512>, 512, c->lds_bytes));
-    c->resident[2] = std::max(nb, 1);
+    const int bss[3] = {128, 256, 512};
+    for (int v = 0; v < NKV; v++)
+        for (int bi = 0; bi < 3; bi++) {
+            int nb = 0;
+            HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_of(v, bss[bi]), bss[bi], c->lds_bytes));
+            c->resident[v][bi] = std::max(nb, 1);
+        }
     c->resident_lds = c->lds_bytes;
     return PT_OK;
 }
 
-// full: launch every resident workgroup x grid_mult (captured runs, where the population
-// is not known on the host); otherwise at most what the last known population needs
-static int launch_advance(pt_ctx* c, bool full) {
-    int rc = ensure_residency(c);
-    if (rc) return rc;
+// work and collision queues of the collision rounds (allocated on first use)
+static int ensure_queues(pt_ctx* c) {
+    if (c->queues) return PT_OK;
+    Queues& Q = c->hp.q;
+    for (int s = 0; s < PT_NSPECIES; s++) {
+        const size_t cap = c->cfg.capacity[s];
+        if (cap == 0) continue;
+        for (int b = 0; b < 2; b++)
+            if (!Q.work[s][b]) {
+                int rc = dalloc(c, (void**)&Q.work[s][b], sizeof(uint32_t) * cap);
+                if (rc) return rc;
+            }
+        for (int k = 0; k < Q.nkq[s]; k++)
+            if (!Q.ev[s][k]) {
+                int rc = dalloc(c, (void**)&Q.ev[s][k], sizeof(uint32_t) * cap);
+                if (rc) return rc;
+            }
+    }
+    c->queues = true;
+    c->dirty = true;
+    return sync_params(c);
+}
+
+// full: launch every resident workgroup x grid_mult (captured runs and queue rounds, whose
+// sizes the host does not know); otherwise at most what the last known population needs
+static int launch_kernel_args(pt_ctx* c, int v, bool full) {
+    void* args[] = {(void*)&c->dp};
     const int bs = c->launch.block;
-    ull blocks = (ull)c->resident[block_index(bs)] * c->ncu * std::max(1, c->launch.grid_mult);
+    ull blocks = (ull)c->resident[v][block_index(bs)] * c->ncu * std::max(1, c->launch.grid_mult);
     if (!full) {
         ull est = c->hn[0] + c->hn[1] + c->hn[2];
         blocks = std::min<ull>(blocks, std::max<ull>((ull)c->ncu, (2 * est + bs - 1) / bs));
     }
-    dim3 g((unsigned)std::max<ull>(blocks, 1));
-    if (bs == 128) hipLaunchKernelGGL(k_advance<128>, g, dim3(128), c->lds_bytes, c->stream, c->dp);
-    else if (bs == 512) hipLaunchKernelGGL(k_advance<512>, g, dim3(512), c->lds_bytes, c->stream, c->dp);
-    else hipLaunchKernelGGL(k_advance<256>, g, dim3(256), c->lds_bytes, c->stream, c->dp);
-    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipLaunchKernel(kernel_of(v, bs), dim3((unsigned)std::max<ull>(blocks, 1)), dim3(bs), args,
+                              (size_t)c->lds_bytes, c->stream));
     return PT_OK;
 }
 
 // one step.  clocked: tfinal and the step index come from the device clock (captured
 // runs); otherwise they are given and the host counts steps.
 static int launch_step(pt_ctx* c, bool clocked, double tfinal) {
+    int rc = ensure_residency(c);
+    if (rc) return rc;
+    const int R = c->launch.rounds;
+    if (R > 0 && (rc = ensure_queues(c))) return rc;
     hipLaunchKernelGGL(k_snapshot, dim3(1), dim3(64), 0, c->stream, c->dp, clocked ? 1 : 0, tfinal, c->step);
     HIPCHK(c, hipGetLastError());
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     const bool prof = c->profile && !clocked;
-    int rc;
     if (prof && (rc = mark(c, &e0))) return rc;
-    if ((rc = launch_advance(c, clocked))) return rc;
+    if (R == 0) {
+        if ((rc = launch_kernel_args(c, KV_RANGE_FULL, clocked))) return rc;
+    } else {
+        if ((rc = launch_kernel_args(c, KV_RANGE_EVENT, clocked))) return rc;
+        for (int r = 1; r <= R; r++) {
+            if ((rc = launch_kernel_args(c, KV_SAMPLE, true))) return rc;
+            hipLaunchKernelGGL(k_rotate, dim3(1), dim3(64), 0, c->stream, c->dp, r == R ? 1 : 0);
+            HIPCHK(c, hipGetLastError());
+            if (r < R && (rc = launch_kernel_args(c, KV_QUEUE_EVENT, true))) return rc;
+        }
+        if ((rc = launch_kernel_args(c, KV_QUEUE_FULL, true))) return rc;
+    }
     if (prof && (rc = mark(c, &e1))) return rc;
     hipLaunchKernelGGL(k_tail, dim3(1), dim3(512), c->lds_bytes, c->stream, c->dp, 100000);
     HIPCHK(c, hipGetLastError());
@@ -1791,14 +2138,22 @@ int pt_run(pt_ctx* c, double t0, double dt, int nsteps, int compact_every, int c
     return PT_OK;
 }
 
+static int apply_launch(pt_ctx* c, const pt_launch& l) {
+    c->launch = l;
+    if (c->hp.defer_mask != l.defer_mask) {
+        c->hp.defer_mask = l.defer_mask;
+        c->dirty = true;
+    }
+    return sync_params(c);
+}
+
 int pt_set_launch(pt_ctx* c, const pt_launch* l) {
     if (!c) return PT_E_ARG;
     if (!l || (l->block != 128 && l->block != 256 && l->block != 512) || l->grid_mult < 1 ||
-        l->grid_mult > 16 || l->compact_every < 1)
-        return fail(c, PT_E_ARG, "launch: block must be 128, 256 or 512, grid_mult in 1..16, compact_every >= 1");
-    c->launch = *l;
-    c->launch.pad = 0;
-    return PT_OK;
+        l->grid_mult > 16 || l->compact_every < 1 || l->rounds < 0 || l->rounds > 8)
+        return fail(c, PT_E_ARG,
+                    "launch: block must be 128, 256 or 512, grid_mult in 1..16, compact_every >= 1, rounds in 0..8");
+    return apply_launch(c, *l);
 }
 
 int pt_get_launch(pt_ctx* c, pt_launch* l) {
@@ -1808,7 +2163,8 @@ int pt_get_launch(pt_ctx* c, pt_launch* l) {
 }
 
 static bool same_launch(const pt_launch& a, const pt_launch& b) {
-    return a.block == b.block && a.grid_mult == b.grid_mult && a.compact_every == b.compact_every;
+    return a.block == b.block && a.grid_mult == b.grid_mult && a.compact_every == b.compact_every &&
+           a.rounds == b.rounds && a.defer_mask == b.defer_mask;
 }
 
 // capture `steps` clocked steps (+ one compaction of every species if `compact`)
@@ -1819,6 +2175,7 @@ static int capture(pt_ctx* c, int which, int steps, bool compact) {
     }
     int rc = ensure_residency(c);
     if (rc) return rc;
+    if (c->launch.rounds > 0 && (rc = ensure_queues(c))) return rc;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int lrc = PT_OK;
     for (int q = 0; q < steps && lrc == PT_OK; q++) lrc = launch_step(c, true, 0.0);
@@ -1940,6 +2297,24 @@ int pt_autotune(pt_ctx* c, double t0, double dt, int trial_steps, pt_tune_trial*
         return PT_OK;
     };
     pt_launch keep = c->launch, top = c->launch;
+    // collision rounds, with the current shape
+    {
+        const int rounds[4] = {0, 2, 3, 4};
+        double best_rate = -1.0;
+        for (int k = 0; k < 4 && rc == PT_OK; k++) {
+            c->launch.rounds = rounds[k];
+            double r = 0;
+            rc = window(trial_steps, 0, &r);
+            if (rc) break;
+            if (trials && nt < max_trials) trials[nt] = pt_tune_trial{c->launch, r};
+            nt++;
+            if (r > best_rate) {
+                best_rate = r;
+                top = c->launch;
+            }
+        }
+        c->launch = top;
+    }
     double top_rate = -1.0;
     const int blocks[3] = {128, 256, 512}, mults[3] = {1, 2, 4};
     for (int bi = 0; bi < 3 && rc == PT_OK; bi++)

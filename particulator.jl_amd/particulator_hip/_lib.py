@@ -77,7 +77,7 @@ class Work(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("block", C.c_int32), ("grid_mult", C.c_int32), ("compact_every", C.c_int32),
-                ("pad", C.c_int32)]
+                ("rounds", C.c_int32), ("defer_mask", C.c_uint32)]
 
 
 class TuneTrial(C.Structure):

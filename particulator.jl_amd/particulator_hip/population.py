@@ -322,14 +322,17 @@ class MultiPopulation:
         check(self.ctx, lib().pt_run_graph(self.ctx, t0, dt, nsteps, check_every))
         return t0 + nsteps * dt
 
-    def set_launch(self, block=256, grid_mult=1, compact_every=8):
-        l = Launch(block, grid_mult, compact_every, 0)
+    def set_launch(self, block=256, grid_mult=1, compact_every=8, rounds=0, defer_mask=1 << 2):
+        """Launch shape, compaction cadence of captured runs, collision rounds and the
+        sampler kinds (bit = pt_proc_kind) that collision rounds queue."""
+        l = Launch(block, grid_mult, compact_every, rounds, defer_mask)
         check(self.ctx, lib().pt_set_launch(self.ctx, C.byref(l)))
 
     def get_launch(self):
         l = Launch()
         check(self.ctx, lib().pt_get_launch(self.ctx, C.byref(l)))
-        return dict(block=l.block, grid_mult=l.grid_mult, compact_every=l.compact_every)
+        return dict(block=l.block, grid_mult=l.grid_mult, compact_every=l.compact_every, rounds=l.rounds,
+                    defer_mask=l.defer_mask)
 
     def autotune(self, t0, dt, trial_steps=4):
         """Time the launch shapes and compaction cadences on the live run and keep the
@@ -339,9 +342,11 @@ class MultiPopulation:
         check(self.ctx, lib().pt_autotune(self.ctx, t0, dt, trial_steps, trials, 32, C.byref(nt),
                                           C.byref(best), C.byref(tend)))
         out = [dict(block=t.launch.block, grid_mult=t.launch.grid_mult,
-                    compact_every=t.launch.compact_every, rate=t.rate) for t in trials[:min(nt.value, 32)]]
-        return (dict(block=best.block, grid_mult=best.grid_mult, compact_every=best.compact_every),
-                out, tend.value)
+                    compact_every=t.launch.compact_every, rounds=t.launch.rounds,
+                    defer_mask=t.launch.defer_mask, rate=t.rate)
+               for t in trials[:min(nt.value, 32)]]
+        return (dict(block=best.block, grid_mult=best.grid_mult, compact_every=best.compact_every,
+                     rounds=best.rounds, defer_mask=best.defer_mask), out, tend.value)
 
     def swarm_stats(self, species, nswarm=None):
         """(nswarm, 4) array: active count, weight, sum of w*E (J), max E (J) per swarm."""

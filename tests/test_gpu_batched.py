@@ -116,7 +116,8 @@ def _pair(seed=5, n_each=20000, nswarm=4):
 def test_graph_replays_equal_eager_steps():
     dt = 1e-12
     eager, graph = _pair()
-    graph.set_launch(block=256, grid_mult=1, compact_every=8)
+    eager.set_launch(block=256, grid_mult=1, compact_every=8, rounds=0)
+    graph.set_launch(block=256, grid_mult=1, compact_every=8, rounds=3, defer_mask=0x7FE)
     t = 0.0
     for nsteps in (8, 16, 11):
         say("eager %d steps from step %d" % (nsteps, eager.step))
@@ -140,9 +141,9 @@ def test_graph_replays_equal_eager_steps():
 def test_launch_shape_does_not_change_particles():
     dt = 1e-12
     a, b = _pair(seed=6, n_each=10000)
-    a.set_launch(block=128, grid_mult=1, compact_every=8)
-    b.set_launch(block=512, grid_mult=4, compact_every=8)
-    say("block 128 x1 vs block 512 x4, 10 steps")
+    a.set_launch(block=128, grid_mult=1, compact_every=8, rounds=0)
+    b.set_launch(block=512, grid_mult=4, compact_every=8, rounds=4, defer_mask=1 << 2)
+    say("block 128 x1 history mode vs block 512 x4 with 4 collision rounds, 10 steps")
     a.run_steps(0.0, dt, 10, compact_every=3)
     b.run_graph(0.0, dt, 10)
     for sp in (0, 1, 2):
@@ -159,8 +160,9 @@ def test_autotune_keeps_the_run_exact():
     say("best %s" % best)
     for tr in trials:
         say("  %s" % tr)
-    nsteps = 9 * 2 + 5 * 16
-    assert len(trials) == 14
+    nsteps = 4 * 2 + 9 * 2 + 5 * 16
+    assert len(trials) == 18
+    assert best["rounds"] in (0, 2, 3, 4)
     assert all(tr["rate"] > 0 for tr in trials)
     assert best["block"] in (128, 256, 512) and best["grid_mult"] in (1, 2, 4)
     assert best["compact_every"] in (1, 2, 4, 8, 16)
