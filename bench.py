@@ -92,8 +92,12 @@ def main():
     ap.add_argument("--defer-mask", type=lambda v: int(v, 0), default=None,
                     help="sampler kinds queued in collision rounds (bit = pt_proc_kind)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=400_000)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure HBM traffic")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-sampler + captured-graph + autotuned variant")
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (to scale the HBM "
                          "byte counters of a profiler) and exit")
@@ -112,6 +116,14 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    traffic = None
+    if not args.no_traffic:
+        # before this process touches the GPU: the counter passes run as child processes
+        try:
+            traffic = pmc_traffic(args)
+        except Exception as e:
+            log("traffic passes failed: %r" % (e,))
 
     import particulator_hip as P
     from particulator_hip import air, co
@@ -174,17 +186,120 @@ def main():
         "config": {"workload": wl["name"], "initial_particles": n, "seed": args.seed,
                    "population_end": counts, "events_per_particle_step": w.events / max(psteps, 1)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "k_advance", "avg_launch_ms": adv_avg_ms,
-                     "algorithmic_bytes_per_launch": adv_bytes},
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic["bytes"] if traffic else None,
+                     "kernel": "k_advance (in-place collisions)", "avg_launch_ms": adv_avg_ms,
+                     "algorithmic_bytes_per_launch": adv_bytes,
+                     "traffic_source": traffic},
     }
+    mp.close()
+    if not args.no_learned:
+        try:
+            out["learned_batched"] = learned_batched(args, wl, n, value)
+        except Exception as e:  # the variant must not hide the plain figure
+            out["learned_batched"] = {"value": None, "error": repr(e)}
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(wl, args.seed, args.cpu_sample, args.cpu_steps)
         except Exception as e:  # the CPU leg must not hide the GPU figure
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
     print(json.dumps(out))
+
+
+FETCH_SCALE = 2.0  # FETCH_SIZE reads 0.5 GiB for a 1 GiB device copy on this GPU
+WRITE_SCALE = 1.0  # WRITE_SIZE reads 1 GiB for the same copy
+
+
+def pmc_traffic(args):
+    """HBM bytes per launch of the step's advance kernel from two rocprofv3 counter passes
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of a short run of this workload, scaled
+    by what the counters read for a plain device copy of known size."""
+    import shutil
+    import sqlite3
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    kib = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="pt_pmc_", dir="/tmp")
+        cmd = [exe, "--pmc", ctr, "-d", d, "-o", "p", "--", sys.executable, os.path.abspath(__file__),
+               "--workload", args.workload, "--steps", "3", "--warmup", "1", "--no-learned",
+               "--no-cpu-baseline", "--no-traffic"] + (["--n", str(args.n)] if args.n else [])
+        log("counter pass %s" % ctr)
+        subprocess.run(cmd, timeout=240, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd="/tmp",
+                       env=dict(os.environ, TMPDIR="/tmp"), check=False)
+        dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("_results.db")]
+        if not dbs:
+            raise RuntimeError("no %s results" % ctr)
+        db = sqlite3.connect(dbs[0])
+        row = db.execute("select avg(value), count(*) from counters_collection where counter_name = ? and "
+                         "kernel_name like '%k_advance<%, 0, false>%'", (ctr,)).fetchone()
+        db.close()
+        shutil.rmtree(d, ignore_errors=True)
+        if not row or row[0] is None:
+            raise RuntimeError("no %s samples for the advance kernel" % ctr)
+        kib[ctr] = (row[0], row[1])
+    fetch = kib["FETCH_SIZE"][0] * 1024 * FETCH_SCALE
+    write = kib["WRITE_SIZE"][0] * 1024 * WRITE_SCALE
+    return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
+            "fetch_kib_raw": kib["FETCH_SIZE"][0], "write_kib_raw": kib["WRITE_SIZE"][0],
+            "dispatches": kib["FETCH_SIZE"][1], "scale": {"FETCH_SIZE": FETCH_SCALE, "WRITE_SIZE": WRITE_SCALE},
+            "note": "counter passes of a 3-step run of the same workload"}
+
+
+def make_world(args, wl, n):
+    import particulator_hip as P
+    from particulator_hip import air, co
+    cut = 1e4 * co.eV
+    te, tg, tp = air.build_tables(air.composition(), wl["efield"], wl["dt"], cut)
+    pops = {}
+    for name, tb, tr in (("electron", te, wl["tracked"][0]), ("photon", tg, wl["tracked"][1]),
+                         ("positron", tp, wl["tracked"][2])):
+        pops[name] = P.Population(wl["cap"], [], tb, cut, tracked=bool(tr))
+    mp = P.MultiPopulation(seed=args.seed, **pops)
+    mp.set_pusher(P.RK2Pusher(P.ElectromagneticField(P.HomogeneousField((0.0, 0.0, -wl["efield"])))))
+    mp.init_particles(P.PT_ELECTRON, n, wl["e1"] * co.eV, (wl["e2"] or wl["e1"]) * co.eV,
+                      loguniform=wl["loguniform"])
+    mp.init()
+    return mp
+
+
+def learned_batched(args, wl, n, plain_value):
+    """The same workload with the learned bremsstrahlung and Moller samplers (trained on the
+    device, gated at KS <= 0.01; their collisions go through one collision round), the
+    launch shape and compaction cadence chosen by the autotuner, and the steps replayed from
+    a captured graph.  Returns its throughput beside the plain figure."""
+    from particulator_hip import learned as LS
+    mp = make_world(args, wl, n)
+    t = time.perf_counter()
+    reps = {LS.NAMES[w]: LS.train(mp, w) for w in (LS.SB, LS.MOLLER)}
+    train_s = time.perf_counter() - t
+    installed = all(r["installed"] for r in reps.values())
+    launch = mp.get_launch()
+    launch.update(rounds=1, defer_mask=(1 << 2) | (1 << 4))
+    mp.set_launch(**launch)
+    dt = wl["dt"]
+    best, trials, tnow = mp.autotune(0.0, dt, trial_steps=2)
+    # the tuner may prefer no collision round; the learned samplers need one
+    best.update(rounds=max(best["rounds"], 1), defer_mask=(1 << 2) | (1 << 4))
+    mp.set_launch(**best)
+    log("learned samplers trained in %.1f s (installed: %s); autotuned launch %s" % (train_s, installed, best))
+    tnow = mp.run_graph(tnow, dt, args.warmup, check_every=max(args.warmup, 1))
+    mp.work(reset=True)
+    mp.profile(True)
+    t0 = time.perf_counter()
+    tnow = mp.run_graph(tnow, dt, args.steps, check_every=args.steps)
+    el = time.perf_counter() - t0
+    w = mp.work(reset=False)
+    value = w.particle_steps / el
+    log("learned+batched: %d steps in %.3f s: %.4g particle-steps/s (%.3fx plain)"
+        % (args.steps, el, value, value / plain_value))
     mp.close()
+    return {"value": value, "unit": "particle-steps/s", "ratio_to_plain": value / plain_value,
+            "ms_per_step": 1e3 * el / args.steps, "launch": best, "learned_installed": installed,
+            "training_s": train_s,
+            "gates": {k: {"ks_max": r["ks_max"], "ks_mean": r["ks_mean"], "ks_noise": r["ks_noise"],
+                          "installed": r["installed"], "samples": r["samples"]} for k, r in reps.items()},
+            "autotune_trials": len(trials)}
 
 
 def check(mp):

@@ -276,6 +276,67 @@ int pt_get_step(pt_ctx* ctx, uint32_t* step);
 int pt_check(pt_ctx* ctx, int64_t* flags);
 int pt_synchronize(pt_ctx* ctx);
 
+/* ---- learned samplers (no reference counterpart) ------------------------------------
+ * Small MLP inverse-CDF samplers replace the rejection samplers of bremsstrahlung
+ * (SeltzerBerger photon energy k/T and modified-Tsai cos theta, src/seltzer.jl:67-122,
+ * src/util.jl:143-159) and of Moller (secondary energy fraction, src/moller.jl:64-87).
+ * Inputs (log10 T, one-hot target Z in {7, 8, 18}, two uniforms) -> 3 hidden layers x 64,
+ * GELU -> outputs (log(k/T), log(1 - cos theta)) or (log(eps/eps0) / log(0.5/eps0)).
+ * Trained on the device in FP32 on quantiles of samples the step's own samplers draw;
+ * evaluated with BF16 weights on the matrix cores (FP32 accumulation) by the sampling
+ * kernel of the collision rounds (collisions of the kind must be in the launch's
+ * defer_mask and rounds >= 1). */
+enum pt_learned { PT_LEARN_SB = 0, PT_LEARN_MOLLER = 1 };
+
+typedef struct {
+    int32_t nenergies;        /* training energies per target Z (log-spaced) */
+    int32_t samples_per_row;  /* exact samples per (energy, Z), a power of two <= 4096 */
+    int32_t quantiles;        /* quantile levels kept per (energy, Z) */
+    int32_t batch;            /* training batch */
+    int32_t steps;            /* Adam steps (cosine-decayed learning rate) */
+    int32_t ks_samples;       /* held-out samples per (energy, Z) of the KS gate */
+    int32_t ks_energies;      /* held-out energies per Z */
+    int32_t pad;
+    double lr;                /* initial learning rate */
+    double emin, emax;        /* energy range of the KS gate (J), and the range in which the
+                                 step uses the network (the rejection sampler elsewhere);
+                                 training covers [emin / 1.25, 1.25 emax] (Moller: above
+                                 2 tcut) */
+    double tcut;              /* Moller secondary-energy cut (J) */
+    double ks_gate;           /* the network is used when the held-out KS <= ks_gate */
+    uint64_t seed;
+} pt_learn_config;
+
+typedef struct {
+    double loss_first;        /* mean batch loss of the first 100 steps (squared quantile
+                                 errors weighted by the inverse squared quantile slope) */
+    double loss_last;         /* mean batch loss of the last 100 steps */
+    double ks_max;            /* held-out two-sample KS, max over (energy, Z, output) */
+    double ks_mean;
+    double ks_noise;          /* 95% KS noise floor of the held-out comparison */
+    double gen_ms, train_ms, eval_ms;
+    int64_t samples;          /* exact samples drawn for the training set */
+    int32_t installed;        /* 1: ks_max <= ks_gate and the network is enabled */
+    int32_t nparams;          /* parameters of the network (inputs in use) */
+} pt_learn_report;
+
+/* fill a config with the defaults for `which` */
+int pt_learn_defaults(int which, pt_learn_config* cfg);
+/* generate the training set, train, build the BF16 image and judge it by the KS gate:
+ * enabled if it passes, otherwise the rejection sampler stays in use (fallback) */
+int pt_learn_train(pt_ctx* ctx, int which, const pt_learn_config* cfg, pt_learn_report* rep);
+/* use (on = 1) or stop using (0) a trained network */
+int pt_learn_enable(pt_ctx* ctx, int which, int on);
+/* read (set = 0) or load (set = 1) the FP32 parameters (n = pt_learn_nparams()); loading
+ * rebuilds the BF16 image and leaves the network disabled */
+int pt_learn_nparams(void);
+int pt_learn_params(pt_ctx* ctx, int which, float* params, int n, int set);
+/* n samples at energies T[i] (J) on targets Z[i]: the network's outputs (learned = 1,
+ * clamped as the step clamps them) or the same transforms of the rejection samplers'
+ * draws (learned = 0); out[2i], out[2i+1] */
+int pt_learn_sample(pt_ctx* ctx, int which, int learned, const double* T, const int32_t* Z, size_t n,
+                    uint64_t seed, double* out);
+
 /* ---- test hooks: same device code as the step, on explicit inputs ------------------ */
 /* Philox4x32-10 on the device: out[4i..4i+3] for counters ctr[4i..] and key. */
 int pt_debug_philox(pt_ctx* ctx, const uint32_t* ctr, const uint32_t key[2], uint32_t* out,
@@ -298,6 +359,13 @@ int pt_debug_collide(pt_ctx* ctx, int species, int proc, size_t n,
                      const double* const soa[PT_NFIELDS], const uint64_t* uid,
                      const double* u, int stride, int32_t* out_kind, int32_t* out_draws,
                      double* out_states, int32_t* out_species, uint64_t* out_uid);
+
+/* one forward/backward pass of the training kernels on a batch: loss and the gradient of
+ * the FP32 parameters (X: B x 16 inputs, Y: B x 2 targets, output weights w0, w1) */
+int pt_debug_mlp_grad(pt_ctx* ctx, const float* params, const float* X, const float* Y, int B, float w0,
+                      float w1, float* grad, float* loss);
+/* matrix-core BF16 inference of the network with parameters `params` on n input rows */
+int pt_debug_mlp_infer(pt_ctx* ctx, const float* params, const float* X, int n, float* Y);
 
 #ifdef __cplusplus
 }

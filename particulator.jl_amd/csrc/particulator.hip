@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "pt_device.h"
+#include "pt_mlp.h"
 
 using namespace pt;
 
@@ -77,6 +78,7 @@ struct Queues {
     uint32_t* ev[PT_NSPECIES][NKQ];
     ull* cnt;  // [3][2] work counts, then [3][NKQ] collision counts
     int8_t kq[PT_NSPECIES][16];  // sampler kind -> collision queue (-1: none)
+    int8_t qkind[PT_NSPECIES][NKQ];  // collision queue -> sampler kind
     int nkq[PT_NSPECIES];
 };
 __host__ __device__ constexpr int QWK(int s, int par) { return 2 * s + par; }
@@ -102,6 +104,8 @@ struct Params {
     Clock* clock;
     Queues q;
     uint32_t defer_mask;  // sampler kinds (bit = pt_proc_kind) queued in collision rounds
+    MlpDev mlp[NLEARN];   // learned samplers (bremsstrahlung, Moller) used by the sampling kernel
+    int lds_mlp_off;      // byte offset of the staged network in the sampling kernel's LDS
     int lds_rate_off[PT_NSPECIES];
     int lds_rb_off[PT_NSPECIES];
     int lds_doubles;
@@ -738,6 +742,10 @@ __global__ void __launch_bounds__(BS) k_sample(const Params* __restrict__ Pp) {
         while (qi + 1 < PT_NSPECIES * NKQ && seg[qi + 1] <= b) qi++;
         const int sp = qi / NKQ, kq = qi % NKQ;
         const ull pos = b - seg[qi] + (ull)lane;
+        const bool valid = pos < P.q.cnt[QEV(sp, kq)];
+        const int kind = P.q.qkind[sp][kq];
+        const int lw = kind == PT_PROC_SB ? LEARN_SB : (kind == PT_PROC_MOLLER ? LEARN_MOLLER : -1);
+        const bool learned = lw >= 0 && P.mlp[lw].on;  // uniform over the wave (one kind per tile)
         ull* c0 = nullptr;
         ull* c1 = nullptr;
         ull* c2 = nullptr;
@@ -745,25 +753,57 @@ __global__ void __launch_bounds__(BS) k_sample(const Params* __restrict__ Pp) {
         uint32_t* a1 = nullptr;
         uint32_t* a2 = nullptr;
         uint32_t v0 = 0, v1 = 0, v2 = 0;
-        if (pos < P.q.cnt[QEV(sp, kq)]) {
-            const SpLite& L = sh.sp[sp];
-            uint32_t ent = P.q.ev[sp][kq][pos];
-            const bool child = (ent & CHILD_BIT) != 0;
-            const ull idx = ent & ~CHILD_BIT;
-            V3 x{L.f[PT_X][idx], L.f[PT_Y][idx], L.f[PT_Z][idx]};
-            V3 p{L.f[PT_PX][idx], L.f[PT_PY][idx], L.f[PT_PZ][idx]};
-            const double w = L.f[PT_W][idx], t = L.f[PT_T][idx];
-            const uint64_t uid = L.uid[idx];
-            int swarm = L.swarm[idx];
+        const SpLite& L = sh.sp[sp];
+        uint32_t ent = 0;
+        ull idx = 0;
+        V3 x{0, 0, 0}, p{1, 0, 0};
+        double w = 0, t = 0, E = 0, pn = 1;
+        uint64_t uid = 0;
+        int swarm = 0, sel = 0;
+        PhiloxStream g;
+        g.init(0, 0, 0);
+        if (valid) {
+            ent = P.q.ev[sp][kq][pos];
+            idx = ent & ~CHILD_BIT;
+            x = V3{L.f[PT_X][idx], L.f[PT_Y][idx], L.f[PT_Z][idx]};
+            p = V3{L.f[PT_PX][idx], L.f[PT_PY][idx], L.f[PT_PZ][idx]};
+            w = L.f[PT_W][idx];
+            t = L.f[PT_T][idx];
+            uid = L.uid[idx];
+            swarm = L.swarm[idx];
             if (swarm < 0 || swarm >= P.nswarm) swarm = 0;
-            const int sel = L.evp[idx];
-            PhiloxStream g;
+            sel = L.evp[idx];
             g.init(P.seed, uid, step);
             g.draw = L.draw[idx];
-            double E = kinenergy(sp, p);
-            double pn = norm3(p);
+            E = kinenergy(sp, p);
+            pn = norm3(p);
+        }
+        float y0 = 0.0f, y1 = 0.0f;
+        // the network is used in the energy range its KS gate covered; outside it the
+        // rejection sampler runs
+        const bool use_net = learned && valid && E >= P.mlp[lw].emin && E <= P.mlp[lw].emax;
+        if (learned) {
+            // the wave's 64 collisions through the network on the matrix cores
+            float xin[MLP_IN];
+            for (int q = 0; q < MLP_IN; q++) xin[q] = 0.0f;
+            if (use_net) {
+                double u1 = g.next(), u2 = g.next();
+                const DevProc& pr = sh.procs[sp][sel];
+                double ph = lw == LEARN_SB ? sb_phase(P.sb[pr.slot].log_energy, P.sb[pr.slot].ne, E) : 0.0;
+                mlp_features(E / EV, pr.Z, u1, u2, ph, xin);
+            }
+            char* ws = (char*)lds + P.lds_mlp_off + (threadIdx.x >> 6) * MLP_WAVE_BYTES;
+            mlp_wave(P.mlp[lw].w, P.mlp[lw].b, (uint16_t*)ws, (float*)(ws + 64 * MLP_H * 2), xin, y0, y1);
+            float b0, b1;
+            mlp_baseline(lw, xin, b0, b1);
+            y0 += b0;
+            y1 += b1;
+        }
+        if (valid) {
+            const bool child = (ent & CHILD_BIT) != 0;
             Ev e;
-            sample_event(sh.procs[sp][sel], sp, p, pn, E, P.sb, P.pe, g, e);
+            if (use_net) learned_event(sh.procs[sp][sel], sp, E, y0, y1, g, e);
+            else sample_event(sh.procs[sp][sel], sp, p, pn, E, P.sb, P.pe, g, e);
             if (e.err) atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
             V3 m0, m1;
             double s0, s1;
@@ -855,6 +895,270 @@ __global__ void __launch_bounds__(512) k_tail(const Params* __restrict__ Pp, int
     // the pass limit was reached with work left: report it
     if (threadIdx.x == 0 && !done) sh.bc.flags[FLAG_TAIL] += 1;
     flush_counters(P, &sh.bc);
+}
+
+// ------------------------------------------------------------------ learned samplers
+// Training data: exact samples of the step's own samplers at rows (T_r, Z_r), transformed
+// to the network's outputs (see pt_mlp.h).  Sample k of row r uses the stream
+// (seed, uid = r * K + k, step = GEN_STEP).
+constexpr uint32_t GEN_STEP = 0xFFFFFFF0u;
+
+__global__ void __launch_bounds__(256) k_mlp_gen(const Params* __restrict__ Pp, int which, const double* __restrict__ T,
+                                                 const int* __restrict__ slot, int nrows, int K, uint64_t seed,
+                                                 double tcut, float* __restrict__ o0, float* __restrict__ o1) {
+    const Params& P = *Pp;
+    ull n = (ull)nrows * K;
+    for (ull q = (ull)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (ull)gridDim.x * blockDim.x) {
+        const int r = (int)(q / K);
+        PhiloxStream g;
+        g.init(seed, q, GEN_STEP);
+        const double eng = T[r];
+        float y0, y1;
+        if (which == LEARN_SB) {
+            double kf = sb_sample(P.sb[slot[r]], eng, g);
+            double cost = sample_tsai(eng, g);
+            y0 = kf > 0.0 ? (float)log(kf) : -100.0f;
+            double omc = 1.0 - cost;
+            y1 = (float)log(omc > 1e-300 ? omc : 1e-300);
+        } else {
+            double e0 = tcut / eng;
+            double eps = moller_sample(tcut, eng, g) / eng;
+            y0 = (float)(log(eps / e0) / log(0.5 / e0));
+            y1 = 0.0f;
+        }
+        o0[q] = y0;
+        o1[q] = y1;
+    }
+}
+
+// sort each row of K floats (K a power of two <= 4096) in LDS: bitonic network
+__global__ void __launch_bounds__(1024) k_mlp_sort_rows(float* __restrict__ data, int K) {
+    __shared__ float v[4096];
+    float* row = data + (size_t)blockIdx.x * K;
+    for (int q = threadIdx.x; q < K; q += blockDim.x) v[q] = row[q];
+    __syncthreads();
+    for (int size = 2; size <= K; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int q = threadIdx.x; q < K / 2; q += blockDim.x) {
+                int lo = 2 * q - (q & (stride - 1));
+                int hi = lo + stride;
+                bool up = (lo & size) == 0;
+                float a = v[lo], b = v[hi];
+                if ((a > b) == up) {
+                    v[lo] = b;
+                    v[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int q = threadIdx.x; q < K; q += blockDim.x) row[q] = v[q];
+}
+
+// quantile table: Q[r][i] = sorted row r at level (i + 1/2) / Kq
+__global__ void k_mlp_quantiles(const float* __restrict__ sorted, int nrows, int K, int Kq, float* __restrict__ Q) {
+    ull n = (ull)nrows * Kq;
+    for (ull q = (ull)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (ull)gridDim.x * blockDim.x) {
+        ull r = q / Kq, i = q % Kq;
+        ull k = (2 * i + 1) * (ull)K / (2 * (ull)Kq);
+        Q[q] = sorted[r * K + k];
+    }
+}
+
+// a training batch: random rows and quantile levels (independent for the two outputs)
+__global__ void k_mlp_batch(int which, const float* __restrict__ Q0, const float* __restrict__ Q1,
+                            const float* __restrict__ rowx, int nrows, int Kq, int B, uint64_t seed,
+                            uint32_t step, float* __restrict__ X, float* __restrict__ Tg) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    uint32_t c0 = (uint32_t)b, c1 = 0x4D4C5000u, c2 = step, c3 = 0;
+    philox10(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+    int r = (int)(((ull)c0 * (ull)nrows) >> 32);
+    int i = (int)(((ull)c1 * (ull)Kq) >> 32);
+    int j = (int)(((ull)c2 * (ull)Kq) >> 32);
+    float* x = X + (size_t)b * MLP_IN;
+    for (int q = 0; q < 4; q++) x[q] = rowx[r * 5 + q];
+    x[4] = 2.0f * ((float)i + 0.5f) / (float)Kq - 1.0f;
+    x[5] = 2.0f * ((float)j + 0.5f) / (float)Kq - 1.0f;
+    x[6] = rowx[r * 5 + 4];
+    for (int q = 7; q < MLP_IN; q++) x[q] = 0.0f;
+    float b0, b1;
+    mlp_baseline(which, x, b0, b1);
+    Tg[2 * b] = Q0[(size_t)r * Kq + i] - b0;
+    Tg[2 * b + 1] = Q1[(size_t)r * Kq + j] - b1;
+}
+
+// dense layer: Z[b][o] = sum_i X[b][i] W[o][i] + bias[o];  H = gelu(Z) (act) or Z
+__global__ void k_mlp_fwd(const float* __restrict__ X, int nin, const float* __restrict__ W,
+                          const float* __restrict__ bias, int nout, int B, int act, float* __restrict__ Z,
+                          float* __restrict__ H) {
+    ull q = (ull)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (ull)B * nout) return;
+    int b = (int)(q / nout), o = (int)(q % nout);
+    const float* x = X + (size_t)b * nin;
+    const float* w = W + (size_t)o * nin;
+    float s = 0.0f;
+    for (int i = 0; i < nin; i++) s = fmaf(x[i], w[i], s);
+    s += bias[o];
+    Z[q] = s;
+    H[q] = act ? gelu(s, false) : s;
+}
+
+// squared-error loss: dY = 2 w_o (Y - T) / B; per-block partial sums of the loss
+__global__ void __launch_bounds__(256) k_mlp_loss(const float* __restrict__ Y, const float* __restrict__ Tg,
+                                                  float w0, float w1, int B, float* __restrict__ dY,
+                                                  float* __restrict__ lpart) {
+    __shared__ float red[256];
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.0f;
+    if (b < B) {
+        float d0 = Y[2 * b] - Tg[2 * b], d1 = Y[2 * b + 1] - Tg[2 * b + 1];
+        dY[2 * b] = 2.0f * w0 * d0 / (float)B;
+        dY[2 * b + 1] = 2.0f * w1 * d1 / (float)B;
+        l = (w0 * d0 * d0 + w1 * d1 * d1) / (float)B;
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lpart[blockIdx.x] = red[0];
+}
+
+// sum of the loss partials of a batch into out[0] (one workgroup, fixed order)
+__global__ void __launch_bounds__(256) k_mlp_sum(const float* __restrict__ part, int n, float* __restrict__ out) {
+    __shared__ float red[256];
+    float s = 0.0f;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) s += part[q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// dZ = dH * gelu'(Z), in place
+__global__ void k_mlp_act_bwd(float* __restrict__ dH, const float* __restrict__ Z, ull n) {
+    ull q = (ull)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dH[q] *= gelu_grad(Z[q]);
+}
+
+// dX[b][i] = sum_o dZ[b][o] W[o][i]
+__global__ void k_mlp_bwd_data(const float* __restrict__ dZ, const float* __restrict__ W, int nin, int nout,
+                               int B, float* __restrict__ dX) {
+    ull q = (ull)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (ull)B * nin) return;
+    int b = (int)(q / nin), i = (int)(q % nin);
+    const float* dz = dZ + (size_t)b * nout;
+    float s = 0.0f;
+    for (int o = 0; o < nout; o++) s = fmaf(dz[o], W[(size_t)o * nin + i], s);
+    dX[q] = s;
+}
+
+// partial weight gradients over a slice of the batch: part[slice][o][i], i == nin: bias
+constexpr int MLP_SLICE = 256;
+__global__ void k_mlp_bwd_w(const float* __restrict__ dZ, const float* __restrict__ X, int nin, int nout, int B,
+                            float* __restrict__ part) {
+    const int slice = blockIdx.x;
+    const int q = blockIdx.y * blockDim.x + threadIdx.x;
+    const int m = nout * (nin + 1);
+    if (q >= m) return;
+    const int o = q / (nin + 1), i = q % (nin + 1);
+    const int b0 = slice * MLP_SLICE, b1 = min(B, b0 + MLP_SLICE);
+    float s = 0.0f;
+    for (int b = b0; b < b1; b++) s = fmaf(dZ[(size_t)b * nout + o], i < nin ? X[(size_t)b * nin + i] : 1.0f, s);
+    part[(size_t)slice * m + q] = s;
+}
+
+// g[dst(q)] = sum over slices of part[s][q], in slice order (deterministic)
+__global__ void k_mlp_reduce(const float* __restrict__ part, int nslices, int nin, int nout, int pw, int pb,
+                             float* __restrict__ g) {
+    const int m = nout * (nin + 1);
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    float s = 0.0f;
+    for (int k = 0; k < nslices; k++) s += part[(size_t)k * m + q];
+    const int o = q / (nin + 1), i = q % (nin + 1);
+    g[i < nin ? pw + o * nin + i : pb + o] = s;
+}
+
+__global__ void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                           float* __restrict__ v, int n, float lr, float b1, float b2, float eps, float c1,
+                           float c2) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    float gq = g[q];
+    float mq = b1 * m[q] + (1.0f - b1) * gq;
+    float vq = b2 * v[q] + (1.0f - b2) * gq * gq;
+    m[q] = mq;
+    v[q] = vq;
+    p[q] -= lr * (mq / c1) / (sqrtf(vq / c2) + eps);
+}
+
+// FP32 parameters -> BF16 inference image
+__global__ void k_mlp_image(const float* __restrict__ p, uint16_t* __restrict__ w, float* __restrict__ b) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < MLP_H * MLP_IN) w[I_W1 + q] = f2bf(p[P_W1 + q]);
+    if (q < MLP_H * MLP_H) {
+        w[I_W2 + q] = f2bf(p[P_W2 + q]);
+        w[I_W3 + q] = f2bf(p[P_W3 + q]);
+    }
+    if (q < MLP_OUTP * MLP_H) w[I_W4 + q] = q < MLP_OUT * MLP_H ? f2bf(p[P_W4 + q]) : (uint16_t)0;
+    if (q < MLP_H) {
+        b[q] = p[P_B1 + q];
+        b[MLP_H + q] = p[P_B2 + q];
+        b[2 * MLP_H + q] = p[P_B3 + q];
+    }
+    if (q < MLP_OUTP) b[3 * MLP_H + q] = q < MLP_OUT ? p[P_B4 + q] : 0.0f;
+}
+
+// matrix-core inference: Y[s] for the n input rows X[s] (64 rows per wave)
+__global__ void __launch_bounds__(256) k_mlp_infer(int which, const uint16_t* __restrict__ wimg,
+                                                   const float* __restrict__ bimg, const float* __restrict__ X, int n,
+                                                   float* __restrict__ Y) {
+    extern __shared__ char smem[];
+    uint16_t* wl = (uint16_t*)smem;
+    float* bl = (float*)(smem + I_NW * 2);
+    char* wave_scratch = smem + MLP_IMAGE_BYTES + (threadIdx.x >> 6) * MLP_WAVE_BYTES;
+    uint16_t* act = (uint16_t*)wave_scratch;
+    float* outb = (float*)(wave_scratch + 64 * MLP_H * 2);
+    MlpDev m{wimg, bimg, 1, 0};
+    mlp_stage(m, wl, bl);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int waves = (gridDim.x * blockDim.x) >> 6;
+    for (int base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64; base < n; base += waves * 64) {
+        int s = base + lane;
+        float x[MLP_IN];
+        for (int q = 0; q < MLP_IN; q++) x[q] = s < n ? X[(size_t)s * MLP_IN + q] : 0.0f;
+        float y0, y1;
+        mlp_wave(wl, bl, act, outb, x, y0, y1);
+        if (s < n) {
+            float b0 = 0.0f, b1 = 0.0f;
+            if (which >= 0) mlp_baseline(which, x, b0, b1);
+            Y[2 * (size_t)s] = y0 + b0;
+            Y[2 * (size_t)s + 1] = y1 + b1;
+        }
+    }
+}
+
+// network inputs for `reps` samples at each of n (T, Z) rows: uniforms from the stream
+// (seed, uid = sample index, GEN_STEP + 1)
+__global__ void k_mlp_features(const Params* __restrict__ Pp, int which, const double* __restrict__ T,
+                               const int32_t* __restrict__ Z, const int32_t* __restrict__ slot, ull n, int reps,
+                               uint64_t seed, float* __restrict__ X) {
+    const Params& P = *Pp;
+    for (ull q = (ull)blockIdx.x * blockDim.x + threadIdx.x; q < n * reps; q += (ull)gridDim.x * blockDim.x) {
+        ull r = q / reps;
+        PhiloxStream g;
+        g.init(seed, q, GEN_STEP + 1);
+        double u1 = g.next(), u2 = g.next();
+        double ph = which == LEARN_SB ? sb_phase(P.sb[slot[r]].log_energy, P.sb[slot[r]].ne, T[r]) : 0.0;
+        mlp_features(T[r] / EV, Z[r], u1, u2, ph, X + q * MLP_IN);
+    }
 }
 
 // init! / advance_init!: r of every active particle (src/mixed_population.jl:20-35)
@@ -1366,6 +1670,18 @@ struct pt_ctx {
     int resident_lds = -1;        // LDS size the residency was computed for
     int stat_cap = 0;             // swarms the per-swarm statistics buffer holds
     double* swarm_part = nullptr;
+    // learned samplers
+    struct Learned {
+        float* params = nullptr;  // FP32 parameters (device)
+        uint16_t* wimg = nullptr;  // BF16 image used by the step
+        float* bimg = nullptr;
+        bool trained = false;
+        bool on = false;
+        double tcut = 1e4 * EV;
+        double emin = 0.0, emax = 0.0;  // energy range the network is used in (its gated range)
+    } learn[NLEARN];
+    int sb_z[MAX_SB] = {0, 0, 0, 0};
+    std::vector<double> sb_le[MAX_SB];  // host copies of the Seltzer-Berger energy grids
     // captured runs: one graph per (steps, compaction) shape
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int gsteps[2] = {0, 0};
@@ -1420,6 +1736,7 @@ static int sync_params(pt_ctx* c) {
     }
     c->hp.lds_doubles = off;
     c->lds_bytes = off * (int)sizeof(double);
+    c->hp.lds_mlp_off = (c->lds_bytes + 15) & ~15;
     HIPCHK(c, hipMemcpyAsync(c->dp, &c->hp, sizeof(Params), hipMemcpyHostToDevice, c->stream));
     c->dirty = false;
     return PT_OK;
@@ -1440,6 +1757,13 @@ struct Scratch {
         if (p) hipFree(p);
     }
 };
+
+template <class T>
+static int dscratch(pt_ctx* c, Scratch<T>& s, size_t count) {
+    hipError_t e = hipMalloc((void**)&s.p, sizeof(T) * (count ? count : 1));
+    if (e != hipSuccess) return fail(c, PT_E_NOMEM, "scratch allocation of %zu bytes failed", sizeof(T) * count);
+    return PT_OK;
+}
 
 // kernel variants: history pass over the step's slots, first collision round, later
 // collision rounds, history pass over the work queues, sampling round
@@ -1663,6 +1987,7 @@ int pt_upload_table(pt_ctx* c, int sp, const pt_table_desc* t) {
         int kd = t->procs[j].kind;
         if (Q.kq[sp][kd] >= 0) continue;
         if (Q.nkq[sp] >= NKQ) return fail(c, PT_E_ARG, "more than %d sampler kinds in one table", NKQ);
+        Q.qkind[sp][Q.nkq[sp]] = (int8_t)kd;
         Q.kq[sp][kd] = (int8_t)Q.nkq[sp]++;
     }
     c->queues = false;  // allocate the new kinds' queues before the next collision round
@@ -1688,7 +2013,8 @@ int pt_upload_sb(pt_ctx* c, int slot, int Z, int ne, int ncum, const double* le,
     c->hp.sb[slot].ncum = ncum;
     c->hp.sb[slot].log_energy = c->dsb_le[slot];
     c->hp.sb[slot].data = c->dsb_data[slot];
-    (void)Z;
+    c->sb_z[slot] = Z;
+    c->sb_le[slot].assign(le, le + ne);
     c->dirty = true;
     rc = sync_params(c);
     if (rc) return rc;
@@ -1940,16 +2266,27 @@ static int mark(pt_ctx* c, hipEvent_t* e) {
 // ---- launch shape
 static int block_index(int bs) { return bs == 128 ? 0 : bs == 512 ? 2 : 1; }
 
+static bool mlp_any(const pt_ctx* c) { return c->hp.mlp[0].on || c->hp.mlp[1].on; }
+
+// dynamic LDS of a kernel variant: the rate tables, and for the sampling kernel with a
+// learned sampler in use, the per-wave scratch of the network
+static size_t kernel_lds(const pt_ctx* c, int v, int bs) {
+    if (v == KV_SAMPLE && mlp_any(c)) return (size_t)c->hp.lds_mlp_off + (size_t)(bs / 64) * MLP_WAVE_BYTES;
+    return (size_t)c->lds_bytes;
+}
+
 static int ensure_residency(pt_ctx* c) {
-    if (c->resident_lds == c->lds_bytes) return PT_OK;
+    const int key = 2 * c->lds_bytes + (mlp_any(c) ? 1 : 0);
+    if (c->resident_lds == key) return PT_OK;
     const int bss[3] = {128, 256, 512};
     for (int v = 0; v < NKV; v++)
         for (int bi = 0; bi < 3; bi++) {
             int nb = 0;
-            HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_of(v, bss[bi]), bss[bi], c->lds_bytes));
+            HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_of(v, bss[bi]), bss[bi],
+                                                                   kernel_lds(c, v, bss[bi])));
             c->resident[v][bi] = std::max(nb, 1);
         }
-    c->resident_lds = c->lds_bytes;
+    c->resident_lds = key;
     return PT_OK;
 }
 
@@ -1987,7 +2324,7 @@ static int launch_kernel_args(pt_ctx* c, int v, bool full) {
         blocks = std::min<ull>(blocks, std::max<ull>((ull)c->ncu, (2 * est + bs - 1) / bs));
     }
     HIPCHK(c, hipLaunchKernel(kernel_of(v, bs), dim3((unsigned)std::max<ull>(blocks, 1)), dim3(bs), args,
-                              (size_t)c->lds_bytes, c->stream));
+                              kernel_lds(c, v, bs), c->stream));
     return PT_OK;
 }
 
@@ -2192,7 +2529,7 @@ static int capture(pt_ctx* c, int which, int steps, bool compact) {
     if (e != hipSuccess) return fail(c, PT_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
     c->gsteps[which] = steps;
     c->glaunch[which] = c->launch;
-    c->glds[which] = c->lds_bytes;
+    c->glds[which] = 2 * c->lds_bytes + (mlp_any(c) ? 1 : 0);
     return PT_OK;
 }
 
@@ -2220,11 +2557,11 @@ int pt_run_graph(pt_ctx* c, double t0, double dt, int nsteps, int check_every) {
     const int ce = c->launch.compact_every;
     const int full = nsteps / ce, rem = nsteps % ce;
     if (full > 0 && (!c->gexec[0] || c->gsteps[0] != ce || !same_launch(c->glaunch[0], c->launch) ||
-                     c->glds[0] != c->lds_bytes)) {
+                     c->glds[0] != 2 * c->lds_bytes + (mlp_any(c) ? 1 : 0))) {
         if ((rc = capture(c, 0, ce, true))) return rc;
     }
     if (rem > 0 && (!c->gexec[1] || c->gsteps[1] != rem || !same_launch(c->glaunch[1], c->launch) ||
-                    c->glds[1] != c->lds_bytes)) {
+                    c->glds[1] != 2 * c->lds_bytes + (mlp_any(c) ? 1 : 0))) {
         if ((rc = capture(c, 1, rem, false))) return rc;
     }
     Clock ck{};
@@ -2635,6 +2972,493 @@ int pt_debug_collide(pt_ctx* c, int sp, int proc, size_t n, const double* const 
     HIPCHK(c, hipMemcpy(out_states, dst.p, sizeof(double) * n * 3 * PT_NFIELDS, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(out_species, dsp.p, sizeof(int32_t) * n * 3, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(out_uid, dou.p, sizeof(uint64_t) * n * 3, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+// ---- learned samplers
+int pt_learn_nparams(void) { return MLP_NPARAM; }
+
+int pt_learn_defaults(int which, pt_learn_config* cfg) {
+    if (!cfg || which < 0 || which >= NLEARN) return PT_E_ARG;
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->nenergies = 8192;
+    cfg->samples_per_row = 4096;
+    cfg->quantiles = 256;
+    cfg->batch = 16384;
+    cfg->steps = 20000;
+    cfg->ks_samples = 400000;
+    cfg->ks_energies = 16;
+    cfg->lr = 2e-3;
+    cfg->tcut = 1e4 * EV;
+    cfg->emin = which == LEARN_SB ? 4e4 * EV : 2.05 * cfg->tcut;
+    cfg->emax = 1e8 * EV;
+    cfg->ks_gate = 0.01;
+    cfg->seed = 0x5EED1234ull;
+    return PT_OK;
+}
+
+static const int LEARN_Z[3] = {7, 8, 18};
+
+// buffers of one training batch
+struct MlpBatch {
+    Scratch<float> X, Tg, Z1, H1, Z2, H2, Z3, H3, Yz, Y, dY, dA, dB, part, lpart;
+    int B = 0, ns = 0, nlb = 0;
+};
+
+static int mlp_batch_alloc(pt_ctx* c, MlpBatch& b, int B) {
+    b.B = B;
+    b.ns = (B + MLP_SLICE - 1) / MLP_SLICE;
+    b.nlb = (B + 255) / 256;
+    const size_t h = (size_t)B * MLP_H;
+    int rc;
+    if ((rc = dscratch(c, b.X, (size_t)B * MLP_IN)) || (rc = dscratch(c, b.Tg, 2 * (size_t)B)) ||
+        (rc = dscratch(c, b.Z1, h)) || (rc = dscratch(c, b.H1, h)) || (rc = dscratch(c, b.Z2, h)) ||
+        (rc = dscratch(c, b.H2, h)) || (rc = dscratch(c, b.Z3, h)) || (rc = dscratch(c, b.H3, h)) ||
+        (rc = dscratch(c, b.Yz, 2 * (size_t)B)) || (rc = dscratch(c, b.Y, 2 * (size_t)B)) ||
+        (rc = dscratch(c, b.dY, 2 * (size_t)B)) || (rc = dscratch(c, b.dA, h)) || (rc = dscratch(c, b.dB, h)) ||
+        (rc = dscratch(c, b.part, (size_t)b.ns * MLP_H * (MLP_H + 1))) || (rc = dscratch(c, b.lpart, b.nlb)))
+        return rc;
+    return PT_OK;
+}
+
+static unsigned nblk(ull n, int bs) { return (unsigned)std::max<ull>(1, (n + bs - 1) / bs); }
+
+// one forward and backward pass on the batch in b.X / b.Tg: gradient g (device), batch loss
+// partials in b.lpart
+static int mlp_fb(pt_ctx* c, MlpBatch& b, const float* P, float w0, float w1, float* g) {
+    const int B = b.B;
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_mlp_fwd, dim3(nblk((ull)B * MLP_H, 256)), dim3(256), 0, st, b.X.p, MLP_IN, P + P_W1, P + P_B1, MLP_H, B, 1, b.Z1.p, b.H1.p);
+    hipLaunchKernelGGL(k_mlp_fwd, dim3(nblk((ull)B * MLP_H, 256)), dim3(256), 0, st, b.H1.p, MLP_H, P + P_W2, P + P_B2, MLP_H, B, 1, b.Z2.p, b.H2.p);
+    hipLaunchKernelGGL(k_mlp_fwd, dim3(nblk((ull)B * MLP_H, 256)), dim3(256), 0, st, b.H2.p, MLP_H, P + P_W3, P + P_B3, MLP_H, B, 1, b.Z3.p, b.H3.p);
+    hipLaunchKernelGGL(k_mlp_fwd, dim3(nblk((ull)B * MLP_OUT, 256)), dim3(256), 0, st, b.H3.p, MLP_H, P + P_W4, P + P_B4, MLP_OUT, B, 0, b.Yz.p, b.Y.p);
+    hipLaunchKernelGGL(k_mlp_loss, dim3(b.nlb), dim3(256), 0, st, b.Y.p, b.Tg.p, w0, w1, B, b.dY.p, b.lpart.p);
+    struct L {
+        const float* dz;
+        const float* in;
+        int nin, nout, pw, pb;
+        float* dnext;  // gradient of the layer's input (null for the first layer)
+        const float* W;
+        const float* zprev;
+    } layers[4] = {
+        {b.dY.p, b.H3.p, MLP_H, MLP_OUT, P_W4, P_B4, b.dA.p, P + P_W4, b.Z3.p},
+        {b.dA.p, b.H2.p, MLP_H, MLP_H, P_W3, P_B3, b.dB.p, P + P_W3, b.Z2.p},
+        {b.dB.p, b.H1.p, MLP_H, MLP_H, P_W2, P_B2, b.dA.p, P + P_W2, b.Z1.p},
+        {b.dA.p, b.X.p, MLP_IN, MLP_H, P_W1, P_B1, nullptr, P + P_W1, nullptr},
+    };
+    for (const L& l : layers) {
+        const int m = l.nout * (l.nin + 1);
+        hipLaunchKernelGGL(k_mlp_bwd_w, dim3(b.ns, nblk(m, 256)), dim3(256), 0, st, l.dz, l.in, l.nin, l.nout, B, b.part.p);
+        hipLaunchKernelGGL(k_mlp_reduce, dim3(nblk(m, 256)), dim3(256), 0, st, b.part.p, b.ns, l.nin, l.nout, l.pw, l.pb, g);
+        if (l.dnext) {
+            hipLaunchKernelGGL(k_mlp_bwd_data, dim3(nblk((ull)B * l.nin, 256)), dim3(256), 0, st, l.dz, l.W, l.nin, l.nout, B, l.dnext);
+            hipLaunchKernelGGL(k_mlp_act_bwd, dim3(nblk((ull)B * l.nin, 256)), dim3(256), 0, st, l.dnext, l.zprev, (ull)B * l.nin);
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    return PT_OK;
+}
+
+// Xavier-uniform weights over the inputs in use, zero biases (host, seeded)
+static void mlp_init(std::vector<float>& p, uint64_t seed) {
+    p.assign(MLP_NPARAM, 0.0f);
+    uint64_t st = seed ^ 0x9E3779B97F4A7C15ull;
+    auto uni = [&]() {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+    };
+    struct {
+        int off, nout, nin, fan_in;
+    } ls[4] = {{P_W1, MLP_H, MLP_IN, 7}, {P_W2, MLP_H, MLP_H, MLP_H}, {P_W3, MLP_H, MLP_H, MLP_H}, {P_W4, MLP_OUT, MLP_H, MLP_H}};
+    for (auto& l : ls) {
+        double a = std::sqrt(6.0 / (l.fan_in + l.nout));
+        for (int o = 0; o < l.nout; o++)
+            for (int i = 0; i < l.nin; i++) p[l.off + o * l.nin + i] = (float)((2.0 * uni() - 1.0) * a);
+    }
+}
+
+static int mlp_image(pt_ctx* c, const float* params, uint16_t* w, float* b) {
+    hipLaunchKernelGGL(k_mlp_image, dim3(nblk(MLP_H * MLP_H, 256)), dim3(256), 0, c->stream, params, w, b);
+    HIPCHK(c, hipGetLastError());
+    return PT_OK;
+}
+
+static int learn_buffers(pt_ctx* c, int which) {
+    pt_ctx::Learned& L = c->learn[which];
+    int rc;
+    if (!L.params && (rc = dalloc(c, (void**)&L.params, sizeof(float) * MLP_NPARAM))) return rc;
+    if (!L.wimg && (rc = dalloc(c, (void**)&L.wimg, sizeof(uint16_t) * I_NW))) return rc;
+    if (!L.bimg && (rc = dalloc(c, (void**)&L.bimg, sizeof(float) * I_NB))) return rc;
+    return PT_OK;
+}
+
+static int set_learn_on(pt_ctx* c, int which, bool on) {
+    pt_ctx::Learned& L = c->learn[which];
+    L.on = on;
+    c->hp.mlp[which].w = L.wimg;
+    c->hp.mlp[which].b = L.bimg;
+    c->hp.mlp[which].emin = L.emin;
+    c->hp.mlp[which].emax = L.emax;
+    c->hp.mlp[which].on = on ? 1 : 0;
+    c->dirty = true;
+    return sync_params(c);
+}
+
+// rows of (energy, Z): slot of the Seltzer-Berger data of each target
+static int learn_rows(pt_ctx* c, int which, int per_z, double emin, double emax, double offset,
+                      std::vector<double>& T, std::vector<int32_t>& Z, std::vector<int32_t>& slot) {
+    T.clear();
+    Z.clear();
+    slot.clear();
+    for (int zi = 0; zi < 3; zi++) {
+        int sl = 0;
+        if (which == LEARN_SB) {
+            sl = -1;
+            for (int q = 0; q < MAX_SB; q++)
+                if (c->sb_z[q] == LEARN_Z[zi] && c->hp.sb[q].data) sl = q;
+            if (sl < 0) return fail(c, PT_E_STATE, "no Seltzer-Berger data uploaded for Z = %d", LEARN_Z[zi]);
+        }
+        for (int m = 0; m < per_z; m++) {
+            double f = (m + offset) / per_z;
+            T.push_back(std::exp(std::log(emin) + f * (std::log(emax) - std::log(emin))));
+            Z.push_back(LEARN_Z[zi]);
+            slot.push_back(sl);
+        }
+    }
+    return PT_OK;
+}
+
+// exact samples (K per row) of the rows, transformed: o0, o1 (device)
+static int learn_exact(pt_ctx* c, int which, const std::vector<double>& T, const std::vector<int32_t>& slot, int K,
+                       uint64_t seed, double tcut, Scratch<float>& o0, Scratch<float>& o1) {
+    const size_t R = T.size();
+    Scratch<double> dT;
+    Scratch<int32_t> ds;
+    int rc;
+    if ((rc = dscratch(c, dT, R)) || (rc = dscratch(c, ds, R)) || (rc = dscratch(c, o0, R * (size_t)K)) ||
+        (rc = dscratch(c, o1, R * (size_t)K)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(dT.p, T.data(), sizeof(double) * R, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ds.p, slot.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, c->stream));
+    if ((rc = sync_params(c))) return rc;
+    hipLaunchKernelGGL(k_mlp_gen, dim3(c->grid), dim3(256), 0, c->stream, c->dp, which, dT.p, ds.p, (int)R, K, seed, tcut,
+                       o0.p, o1.p);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+// network outputs for `reps` samples at each row (device Y, 2 per sample)
+static int learn_infer(pt_ctx* c, int which, const uint16_t* wimg, const float* bimg, const std::vector<double>& T,
+                       const std::vector<int32_t>& Z, const std::vector<int32_t>& slot, int reps, uint64_t seed,
+                       Scratch<float>& Y) {
+    const size_t R = T.size(), n = R * (size_t)reps;
+    Scratch<double> dT;
+    Scratch<int32_t> dZ, ds;
+    Scratch<float> X;
+    int rc;
+    if ((rc = dscratch(c, dT, R)) || (rc = dscratch(c, dZ, R)) || (rc = dscratch(c, ds, R)) ||
+        (rc = dscratch(c, X, n * MLP_IN)) || (rc = dscratch(c, Y, 2 * n)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(dT.p, T.data(), sizeof(double) * R, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dZ.p, Z.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ds.p, slot.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, c->stream));
+    if ((rc = sync_params(c))) return rc;
+    hipLaunchKernelGGL(k_mlp_features, dim3(c->grid), dim3(256), 0, c->stream, c->dp, which, dT.p, dZ.p, ds.p, (ull)R,
+                       reps, seed, X.p);
+    const size_t lds = MLP_IMAGE_BYTES + 4 * MLP_WAVE_BYTES;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(c->grid), dim3(256), lds, c->stream, which, wimg, bimg, X.p, (int)n, Y.p);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+// the clamps the step applies to the outputs
+static void learn_clamp(int which, float& y0, float& y1) {
+    if (which == LEARN_SB) {
+        y0 = std::min(y0, 0.0f);
+        y1 = std::min(y1, 0.6931472f);
+    } else {
+        y0 = std::min(std::max(y0, 0.0f), 1.0f);
+        y1 = 0.0f;
+    }
+}
+
+static double ks2(std::vector<float>& a, std::vector<float>& b) {
+    std::sort(a.begin(), a.end());
+    std::sort(b.begin(), b.end());
+    size_t i = 0, j = 0;
+    double d = 0.0;
+    while (i < a.size() && j < b.size()) {
+        float v = std::min(a[i], b[j]);
+        while (i < a.size() && a[i] <= v) i++;
+        while (j < b.size() && b[j] <= v) j++;
+        d = std::max(d, std::fabs((double)i / a.size() - (double)j / b.size()));
+    }
+    return d;
+}
+
+int pt_learn_train(pt_ctx* c, int which, const pt_learn_config* cfg, pt_learn_report* rep) {
+    if (!c) return PT_E_ARG;
+    if (which < 0 || which >= NLEARN || !cfg) return fail(c, PT_E_ARG, "bad learned-sampler arguments");
+    const int M = cfg->nenergies, K = cfg->samples_per_row, Kq = cfg->quantiles, B = cfg->batch;
+    if (M < 1 || K < 2 || K > 4096 || (K & (K - 1)) || Kq < 2 || Kq > K || B < 256 || cfg->steps < 1 ||
+        cfg->ks_samples < 100 || cfg->ks_energies < 1 || !(cfg->emin > 0) || !(cfg->emax > cfg->emin) ||
+        (which == LEARN_MOLLER && !(cfg->emin > 2.0 * cfg->tcut)))
+        return fail(c, PT_E_ARG, "bad learned-sampler configuration");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = learn_buffers(c, which))) return rc;
+    pt_ctx::Learned& Ln = c->learn[which];
+    Ln.tcut = cfg->tcut;
+    Ln.emin = cfg->emin;
+    Ln.emax = cfg->emax;
+    pt_learn_report r{};
+    // 1. training set: exact samples -> per-row quantiles
+    std::vector<double> T;
+    std::vector<int32_t> Z, slot;
+    // the training energies reach a little beyond the gated range, so that the range's
+    // edges are not the network's edges
+    const double tlo = which == LEARN_MOLLER ? std::max(cfg->emin / 1.25, 2.0 * cfg->tcut * 1.0001) : cfg->emin / 1.25;
+    if ((rc = learn_rows(c, which, M, tlo, cfg->emax * 1.25, 0.5, T, Z, slot))) return rc;
+    const int R = (int)T.size();
+    Scratch<float> Q0, Q1, rowx;
+    if ((rc = dscratch(c, Q0, (size_t)R * Kq)) || (rc = dscratch(c, Q1, (size_t)R * Kq)) ||
+        (rc = dscratch(c, rowx, (size_t)R * 5)))
+        return rc;
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    {
+        Scratch<float> o0, o1;
+        if ((rc = learn_exact(c, which, T, slot, K, cfg->seed, cfg->tcut, o0, o1))) return rc;
+        hipLaunchKernelGGL(k_mlp_sort_rows, dim3(R), dim3(1024), 0, c->stream, o0.p, K);
+        hipLaunchKernelGGL(k_mlp_sort_rows, dim3(R), dim3(1024), 0, c->stream, o1.p, K);
+        hipLaunchKernelGGL(k_mlp_quantiles, dim3(c->grid), dim3(256), 0, c->stream, o0.p, R, K, Kq, Q0.p);
+        hipLaunchKernelGGL(k_mlp_quantiles, dim3(c->grid), dim3(256), 0, c->stream, o1.p, R, K, Kq, Q1.p);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    std::vector<float> hx((size_t)R * 5);
+    for (int q = 0; q < R; q++) {
+        float x[MLP_IN];
+        double ph = 0.0;
+        if (which == LEARN_SB) {
+            const std::vector<double>& le = c->sb_le[slot[q]];
+            ph = sb_phase(le.data(), (int)le.size(), T[q]);
+        }
+        mlp_features(T[q] / EV, Z[q], 0.5, 0.5, ph, x);
+        for (int k = 0; k < 4; k++) hx[(size_t)q * 5 + k] = x[k];
+        hx[(size_t)q * 5 + 4] = x[6];
+    }
+    HIPCHK(c, hipMemcpyAsync(rowx.p, hx.data(), sizeof(float) * hx.size(), hipMemcpyHostToDevice, c->stream));
+    r.samples = (int64_t)R * K;
+    // 2. training
+    std::vector<float> p0;
+    mlp_init(p0, cfg->seed);
+    HIPCHK(c, hipMemcpyAsync(Ln.params, p0.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, c->stream));
+    MlpBatch b;
+    if ((rc = mlp_batch_alloc(c, b, B))) return rc;
+    Scratch<float> g, m, v, lossv;
+    if ((rc = dscratch(c, g, MLP_NPARAM)) || (rc = dscratch(c, m, MLP_NPARAM)) || (rc = dscratch(c, v, MLP_NPARAM)) ||
+        (rc = dscratch(c, lossv, (size_t)cfg->steps)))
+        return rc;
+    HIPCHK(c, hipMemsetAsync(m.p, 0, sizeof(float) * MLP_NPARAM, c->stream));
+    HIPCHK(c, hipMemsetAsync(v.p, 0, sizeof(float) * MLP_NPARAM, c->stream));
+    const float w0 = 1.0f, w1 = which == LEARN_SB ? 1.0f : 0.0f;
+    double c1 = 1.0, c2 = 1.0;
+    for (int it = 0; it < cfg->steps; it++) {
+        hipLaunchKernelGGL(k_mlp_batch, dim3(nblk(B, 256)), dim3(256), 0, c->stream, which, Q0.p, Q1.p, rowx.p, R, Kq, B,
+                           cfg->seed ^ 0xBA7C4ull, (uint32_t)it, b.X.p, b.Tg.p);
+        if ((rc = mlp_fb(c, b, Ln.params, w0, w1, g.p))) return rc;
+        hipLaunchKernelGGL(k_mlp_sum, dim3(1), dim3(256), 0, c->stream, b.lpart.p, b.nlb, lossv.p + it);
+        c1 *= 0.9;
+        c2 *= 0.999;
+        double lr = cfg->lr * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(M_PI * it / cfg->steps)));
+        hipLaunchKernelGGL(k_mlp_adam, dim3(nblk(MLP_NPARAM, 256)), dim3(256), 0, c->stream, Ln.params, g.p, m.p, v.p,
+                           MLP_NPARAM, (float)lr, 0.9f, 0.999f, 1e-8f, (float)(1.0 - c1), (float)(1.0 - c2));
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    std::vector<float> hl(cfg->steps);
+    HIPCHK(c, hipMemcpyAsync(hl.data(), lossv.p, sizeof(float) * hl.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int nav = std::min(100, cfg->steps);
+    for (int q = 0; q < nav; q++) {
+        r.loss_first += hl[q] / nav;
+        r.loss_last += hl[cfg->steps - 1 - q] / nav;
+    }
+    if ((rc = mlp_image(c, Ln.params, Ln.wimg, Ln.bimg))) return rc;
+    Ln.trained = true;
+    // 3. KS gate on held-out energies: rejection sampler vs network
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    std::vector<double> T2;
+    std::vector<int32_t> Z2, slot2;
+    if ((rc = learn_rows(c, which, cfg->ks_energies, cfg->emin, cfg->emax, 0.37, T2, Z2, slot2))) return rc;
+    const int n = cfg->ks_samples;
+    const size_t R2 = T2.size();
+    std::vector<float> e0(R2 * (size_t)n), e1(R2 * (size_t)n), y(2 * R2 * (size_t)n);
+    {
+        Scratch<float> o0, o1, Y;
+        if ((rc = learn_exact(c, which, T2, slot2, n, cfg->seed ^ 0x4B5ull, cfg->tcut, o0, o1))) return rc;
+        if ((rc = learn_infer(c, which, Ln.wimg, Ln.bimg, T2, Z2, slot2, n, cfg->seed ^ 0x1F3ull, Y))) return rc;
+        HIPCHK(c, hipMemcpy(e0.data(), o0.p, sizeof(float) * e0.size(), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(e1.data(), o1.p, sizeof(float) * e1.size(), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(y.data(), Y.p, sizeof(float) * y.size(), hipMemcpyDeviceToHost));
+    }
+    double ksmax = 0.0, kssum = 0.0;
+    int nks = 0;
+    const int nout = which == LEARN_SB ? 2 : 1;
+    for (size_t row = 0; row < R2; row++) {
+        for (int o = 0; o < nout; o++) {
+            std::vector<float> a(n), bb(n);
+            for (int q = 0; q < n; q++) {
+                size_t s = row * n + q;
+                a[q] = o == 0 ? e0[s] : e1[s];
+                float y0 = y[2 * s], y1 = y[2 * s + 1];
+                learn_clamp(which, y0, y1);
+                bb[q] = o == 0 ? y0 : y1;
+            }
+            double d = ks2(a, bb);
+            ksmax = std::max(ksmax, d);
+            kssum += d;
+            nks++;
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[4]));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    r.gen_ms = ms;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
+    r.train_ms = ms;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
+    r.eval_ms = ms;
+    r.ks_max = ksmax;
+    r.ks_mean = kssum / std::max(nks, 1);
+    r.ks_noise = 1.36 * std::sqrt(2.0 / n);
+    r.nparams = MLP_H * (which == LEARN_SB ? 7 : 6) + MLP_H + 2 * (MLP_H * MLP_H + MLP_H) + (which == LEARN_SB ? 2 : 1) * (MLP_H + 1);
+    r.installed = ksmax <= cfg->ks_gate ? 1 : 0;
+    if ((rc = set_learn_on(c, which, r.installed != 0))) return rc;
+    c->resident_lds = -1;
+    if (rep) *rep = r;
+    return PT_OK;
+}
+
+int pt_learn_enable(pt_ctx* c, int which, int on) {
+    if (!c) return PT_E_ARG;
+    if (which < 0 || which >= NLEARN) return fail(c, PT_E_ARG, "bad learned sampler %d", which);
+    if (on && !c->learn[which].trained) return fail(c, PT_E_STATE, "learned sampler %d is not trained", which);
+    HIPCHK(c, hipSetDevice(c->device));
+    c->resident_lds = -1;
+    return set_learn_on(c, which, on != 0);
+}
+
+int pt_learn_params(pt_ctx* c, int which, float* params, int n, int set) {
+    if (!c) return PT_E_ARG;
+    if (which < 0 || which >= NLEARN || !params || n != MLP_NPARAM) return fail(c, PT_E_ARG, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = learn_buffers(c, which);
+    if (rc) return rc;
+    pt_ctx::Learned& L = c->learn[which];
+    if (!set) {
+        if (!L.trained) return fail(c, PT_E_STATE, "learned sampler %d has no parameters", which);
+        HIPCHK(c, hipMemcpy(params, L.params, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
+        return PT_OK;
+    }
+    HIPCHK(c, hipMemcpyAsync(L.params, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, c->stream));
+    if ((rc = mlp_image(c, L.params, L.wimg, L.bimg))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    L.trained = true;
+    c->resident_lds = -1;
+    return set_learn_on(c, which, false);
+}
+
+int pt_learn_sample(pt_ctx* c, int which, int learned, const double* T, const int32_t* Z, size_t n, uint64_t seed,
+                    double* out) {
+    if (!c) return PT_E_ARG;
+    if (which < 0 || which >= NLEARN || !T || !Z || !out) return fail(c, PT_E_ARG, "bad arguments");
+    if (learned && !c->learn[which].trained) return fail(c, PT_E_STATE, "learned sampler %d is not trained", which);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (n == 0) return PT_OK;
+    std::vector<double> Tv(T, T + n);
+    std::vector<int32_t> Zv(Z, Z + n), slot(n, 0);
+    for (size_t q = 0; q < n; q++) {
+        if (which == LEARN_SB) {
+            slot[q] = -1;
+            for (int s = 0; s < MAX_SB; s++)
+                if (c->sb_z[s] == Z[q] && c->hp.sb[s].data) slot[q] = s;
+            if (slot[q] < 0) return fail(c, PT_E_STATE, "no Seltzer-Berger data uploaded for Z = %d", Z[q]);
+        } else if (!(T[q] > 2.0 * c->learn[which].tcut)) {
+            return fail(c, PT_E_ARG, "Moller energy %g J not above 2 tcut", T[q]);
+        }
+    }
+    std::vector<float> y(2 * n);
+    int rc;
+    if (learned) {
+        Scratch<float> Y;
+        const pt_ctx::Learned& L = c->learn[which];
+        if ((rc = learn_infer(c, which, L.wimg, L.bimg, Tv, Zv, slot, 1, seed, Y))) return rc;
+        HIPCHK(c, hipMemcpy(y.data(), Y.p, sizeof(float) * 2 * n, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n; q++) learn_clamp(which, y[2 * q], y[2 * q + 1]);
+    } else {
+        Scratch<float> o0, o1;
+        if ((rc = learn_exact(c, which, Tv, slot, 1, seed, c->learn[which].tcut, o0, o1))) return rc;
+        std::vector<float> a(n), b(n);
+        HIPCHK(c, hipMemcpy(a.data(), o0.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(b.data(), o1.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n; q++) {
+            y[2 * q] = a[q];
+            y[2 * q + 1] = b[q];
+        }
+    }
+    for (size_t q = 0; q < 2 * n; q++) out[q] = y[q];
+    return PT_OK;
+}
+
+int pt_debug_mlp_grad(pt_ctx* c, const float* params, const float* X, const float* Y, int B, float w0, float w1,
+                      float* grad, float* loss) {
+    if (!c) return PT_E_ARG;
+    if (!params || !X || !Y || !grad || !loss || B < 1) return fail(c, PT_E_ARG, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    MlpBatch b;
+    Scratch<float> P, g;
+    int rc;
+    if ((rc = mlp_batch_alloc(c, b, B)) || (rc = dscratch(c, P, MLP_NPARAM)) || (rc = dscratch(c, g, MLP_NPARAM)))
+        return rc;
+    HIPCHK(c, hipMemcpy(P.p, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b.X.p, X, sizeof(float) * (size_t)B * MLP_IN, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b.Tg.p, Y, sizeof(float) * 2 * (size_t)B, hipMemcpyHostToDevice));
+    if ((rc = mlp_fb(c, b, P.p, w0, w1, g.p))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<float> lp(b.nlb);
+    HIPCHK(c, hipMemcpy(lp.data(), b.lpart.p, sizeof(float) * lp.size(), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(grad, g.p, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
+    double s = 0.0;
+    for (float v : lp) s += v;
+    *loss = (float)s;
+    return PT_OK;
+}
+
+int pt_debug_mlp_infer(pt_ctx* c, const float* params, const float* X, int n, float* Y) {
+    if (!c) return PT_E_ARG;
+    if (!params || !X || !Y || n < 1) return fail(c, PT_E_ARG, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    Scratch<float> P, bimg, dX, dY;
+    Scratch<uint16_t> wimg;
+    int rc;
+    if ((rc = dscratch(c, P, MLP_NPARAM)) || (rc = dscratch(c, wimg, I_NW)) || (rc = dscratch(c, bimg, I_NB)) ||
+        (rc = dscratch(c, dX, (size_t)n * MLP_IN)) || (rc = dscratch(c, dY, 2 * (size_t)n)))
+        return rc;
+    HIPCHK(c, hipMemcpy(P.p, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dX.p, X, sizeof(float) * (size_t)n * MLP_IN, hipMemcpyHostToDevice));
+    if ((rc = mlp_image(c, P.p, wimg.p, bimg.p))) return rc;
+    const size_t lds = MLP_IMAGE_BYTES + 4 * MLP_WAVE_BYTES;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(nblk(n, 256)), dim3(256), lds, c->stream, -1, wimg.p, bimg.p, dX.p, n, dY.p);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(Y, dY.p, sizeof(float) * 2 * (size_t)n, hipMemcpyDeviceToHost));
     return PT_OK;
 }
 

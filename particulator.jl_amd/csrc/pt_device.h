@@ -419,6 +419,23 @@ __device__ __forceinline__ double sb_sample(const DevSB& sb, double T, G& g) {
     return exp(S / A);
 }
 
+// Position of log T inside its cell of the Seltzer-Berger energy grid, in (0, 1]: the cell
+// sb_sample interpolates in (the same search).  Its interpolation swaps the energy corners
+// (src/seltzer.jl:116-119), so the sampled distribution jumps at every grid node; a learned
+// sampler gets this phase as an input to follow those jumps.
+__host__ __device__ inline double sb_phase(const double* log_energy, int ne, double T) {
+    double y = log(T);
+    int lo = 0, hi = ne;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (log_energy[mid] < y) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < 1 || lo >= ne) return 0.0;
+    double y1 = log_energy[lo - 1], y2 = log_energy[lo];
+    return (y - y1) / (y2 - y1);
+}
+
 // sample_secondary_energy_and_cos_theta(::Compton), src/compton.jl:119-144
 template <class G>
 __device__ __forceinline__ void compton_sample(double eng, G& g, double& E1, double& cost) {
@@ -753,6 +770,55 @@ __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3
         e.src0 = SRC_A;
         e.sp0 = sp;
         break;
+    }
+}
+
+// The collision of a bremsstrahlung or Moller process from the outputs of its learned
+// sampler (y0, y1: see pt_mlp.h) instead of the rejection samplers; the kinematics and the
+// remaining draws (azimuth, constructors) are those of sample_event.
+template <class G>
+__device__ __forceinline__ void learned_event(const DevProc& pr, int sp, double eng, float y0, float y1, G& g, Ev& e) {
+    e.err = 0;
+    e.nturn = 1;
+    e.sgnB = 1;
+    e.cB = 0.0;
+    e.nB = 0.0;
+    e.src1 = SRC_A;
+    e.sp1 = sp;
+    if (pr.kind == PT_PROC_SB) {
+        double kf = exp((double)(y0 < 0.0f ? y0 : 0.0f));
+        double k = eng * kf;
+        double omc = exp((double)(y1 < 0.6931472f ? y1 : 0.6931472f));
+        double cost = 1.0 - omc;
+        e.nA = k / C;
+        e.cA = cost < -1.0 ? -1.0 : cost;
+        e.phi = TWO_PI * g.next();
+        e.keep = 1;
+        e.ne = 2;
+        e.src0 = SRC_PMA;
+        e.sp0 = PT_ELECTRON;
+        e.src1 = SRC_A;
+        e.sp1 = PT_PHOTON;
+        if (!(k < eng)) e.err = 2;
+    } else {  // Moller
+        double E0 = eng;
+        double e0 = pr.tcut / E0;
+        double y = y0 < 0.0f ? 0.0 : (y0 > 1.0f ? 1.0 : (double)y0);
+        double eps = e0 * exp(y * log(0.5 / e0));
+        double E2 = eps * E0, E1 = E0 - E2;
+        e.phi = TWO_PI * g.next();
+        e.nA = sqrt(E1 * E1 + 2.0 * MC2 * E1) / C;
+        e.nB = sqrt(E2 * E2 + 2.0 * MC2 * E2) / C;
+        e.cA = sqrt(E1 * (E0 + 2.0 * MC2) / (E0 * (E1 + 2.0 * MC2)));
+        e.cB = sqrt(E2 * (E0 + 2.0 * MC2) / (E0 * (E2 + 2.0 * MC2)));
+        e.nturn = 2;
+        e.sgnB = -1;
+        e.keep = 1;
+        e.ne = 2;
+        e.src0 = SRC_A;
+        e.sp0 = sp;
+        e.src1 = SRC_B;
+        e.sp1 = PT_ELECTRON;
     }
 }
 

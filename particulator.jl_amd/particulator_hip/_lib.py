@@ -80,11 +80,26 @@ class Launch(C.Structure):
                 ("rounds", C.c_int32), ("defer_mask", C.c_uint32)]
 
 
+class LearnConfig(C.Structure):
+    _fields_ = [("nenergies", C.c_int32), ("samples_per_row", C.c_int32), ("quantiles", C.c_int32),
+                ("batch", C.c_int32), ("steps", C.c_int32), ("ks_samples", C.c_int32),
+                ("ks_energies", C.c_int32), ("pad", C.c_int32), ("lr", C.c_double), ("emin", C.c_double),
+                ("emax", C.c_double), ("tcut", C.c_double), ("ks_gate", C.c_double), ("seed", C.c_uint64)]
+
+
+class LearnReport(C.Structure):
+    _fields_ = [("loss_first", C.c_double), ("loss_last", C.c_double), ("ks_max", C.c_double),
+                ("ks_mean", C.c_double), ("ks_noise", C.c_double), ("gen_ms", C.c_double),
+                ("train_ms", C.c_double), ("eval_ms", C.c_double), ("samples", C.c_int64),
+                ("installed", C.c_int32), ("nparams", C.c_int32)]
+
+
 class TuneTrial(C.Structure):
     _fields_ = [("launch", Launch), ("rate", C.c_double)]
 
 
 DP = C.POINTER(C.c_double)
+FP = C.POINTER(C.c_float)
 SOA = DP * NFIELDS
 VP = C.c_void_p
 
@@ -118,6 +133,14 @@ SIGNATURES = {
     "pt_autotune": (C.c_int, [VP, C.c_double, C.c_double, C.c_int, C.POINTER(TuneTrial), C.c_int,
                               C.POINTER(C.c_int), C.POINTER(Launch), DP]),
     "pt_swarm_stats": (C.c_int, [VP, C.c_int, C.c_int, DP]),
+    "pt_learn_defaults": (C.c_int, [C.c_int, C.POINTER(LearnConfig)]),
+    "pt_learn_train": (C.c_int, [VP, C.c_int, C.POINTER(LearnConfig), C.POINTER(LearnReport)]),
+    "pt_learn_enable": (C.c_int, [VP, C.c_int, C.c_int]),
+    "pt_learn_nparams": (C.c_int, []),
+    "pt_learn_params": (C.c_int, [VP, C.c_int, FP, C.c_int, C.c_int]),
+    "pt_learn_sample": (C.c_int, [VP, C.c_int, C.c_int, DP, C.POINTER(C.c_int32), C.c_size_t, C.c_uint64, DP]),
+    "pt_debug_mlp_grad": (C.c_int, [VP, FP, FP, FP, C.c_int, C.c_float, C.c_float, FP, FP]),
+    "pt_debug_mlp_infer": (C.c_int, [VP, FP, FP, C.c_int, FP]),
     "pt_collision_counts": (C.c_int, [VP, C.c_int, C.POINTER(C.c_int64), C.c_int, C.c_int]),
     "pt_tally_read": (C.c_int, [VP, C.c_int, DP, C.c_int, DP, DP]),
     "pt_work_read": (C.c_int, [VP, C.POINTER(Work), C.c_int]),

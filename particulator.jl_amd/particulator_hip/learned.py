@@ -1,0 +1,91 @@
+"""Learned inverse-CDF samplers (no counterpart in the reference): small MLPs that replace
+the rejection samplers of bremsstrahlung (SeltzerBerger k/T and the modified-Tsai angle,
+src/seltzer.jl:67-122, src/util.jl:143-159) and Moller (src/moller.jl:64-87) inside the
+device step.  Training, the KS gate with fallback to the rejection samplers, and matrix-core
+inference all run in the library; this module is its Python face.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import LearnConfig, LearnReport, check, lib
+
+SB, MOLLER = 0, 1
+NAMES = {SB: "bremsstrahlung", MOLLER: "moller"}
+
+
+def nparams():
+    return int(lib().pt_learn_nparams())
+
+
+def defaults(which):
+    cfg = LearnConfig()
+    rc = lib().pt_learn_defaults(which, C.byref(cfg))
+    if rc != 0:
+        raise ValueError("bad learned sampler %r" % which)
+    return {f: getattr(cfg, f) for f, _ in LearnConfig._fields_ if f != "pad"}
+
+
+def train(mp, which, **overrides):
+    """Train the sampler on the device from the context's own samplers and judge it by the
+    held-out KS gate; it is used by the step if it passes.  Returns the report as a dict."""
+    d = defaults(which)
+    d.update(overrides)
+    cfg = LearnConfig()
+    for k, v in d.items():
+        setattr(cfg, k, v)
+    rep = LearnReport()
+    check(mp.ctx, lib().pt_learn_train(mp.ctx, which, C.byref(cfg), C.byref(rep)))
+    return {f: getattr(rep, f) for f, _ in LearnReport._fields_}
+
+
+def enable(mp, which, on=True):
+    check(mp.ctx, lib().pt_learn_enable(mp.ctx, which, int(on)))
+
+
+def get_params(mp, which):
+    p = np.zeros(nparams(), dtype=np.float32)
+    check(mp.ctx, lib().pt_learn_params(mp.ctx, which, p.ctypes.data_as(C.POINTER(C.c_float)), p.size, 0))
+    return p
+
+
+def set_params(mp, which, p):
+    p = np.ascontiguousarray(p, dtype=np.float32)
+    check(mp.ctx, lib().pt_learn_params(mp.ctx, which, p.ctypes.data_as(C.POINTER(C.c_float)), p.size, 1))
+
+
+def sample(mp, which, learned, T, Z, seed=1):
+    """(n, 2) transformed draws at energies T (J) on targets Z: the network's (learned) or
+    the rejection samplers'."""
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    Z = np.ascontiguousarray(np.broadcast_to(Z, T.shape), dtype=np.int32)
+    out = np.zeros(2 * T.size)
+    check(mp.ctx, lib().pt_learn_sample(mp.ctx, which, int(learned), T.ctypes.data_as(C.POINTER(C.c_double)),
+                                        Z.ctypes.data_as(C.POINTER(C.c_int32)), T.size, seed,
+                                        out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out.reshape(-1, 2)
+
+
+def debug_grad(mp, params, X, Y, w0=1.0, w1=1.0):
+    """loss and gradient of one pass of the device training kernels"""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.ascontiguousarray(Y, dtype=np.float32)
+    g = np.zeros(nparams(), dtype=np.float32)
+    loss = C.c_float()
+    fp = C.POINTER(C.c_float)
+    check(mp.ctx, lib().pt_debug_mlp_grad(mp.ctx, params.ctypes.data_as(fp), X.ctypes.data_as(fp),
+                                          Y.ctypes.data_as(fp), X.shape[0], w0, w1, g.ctypes.data_as(fp),
+                                          C.byref(loss)))
+    return loss.value, g
+
+
+def debug_infer(mp, params, X):
+    """the matrix-core BF16 inference kernel on explicit inputs"""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.zeros((X.shape[0], 2), dtype=np.float32)
+    fp = C.POINTER(C.c_float)
+    check(mp.ctx, lib().pt_debug_mlp_infer(mp.ctx, params.ctypes.data_as(fp), X.ctypes.data_as(fp), X.shape[0],
+                                           Y.ctypes.data_as(fp)))
+    return Y

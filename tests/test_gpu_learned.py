@@ -1,0 +1,168 @@
+"""Learned samplers on the device: the training kernels' gradients and the matrix-core
+inference against the CPU restatement (oracle/mlp.py), training with the KS gate and its
+fallback, and the step with the learned bremsstrahlung sampler against the plain step.
+
+Bars: gradients to FP32 accumulation (1e-4 of the largest component); BF16 inference to the
+BF16 emulation of the oracle (bf16 rounding flips only); held-out KS <= 0.01 to install a
+network; end-to-end electron spectra of the learned step vs the plain step KS <= 0.01.
+"""
+import math
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle as O
+from particulator_hip import co, learned as LS
+from particulator_hip.population import kinenergy
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def mp():
+    te, tg, tp = H.tables()
+    m = H.make_device(te, tg, tp, 42, caps=(1 << 20, 1 << 16, 1 << 16), tracked=(True, False, False))
+    yield m
+    m.close()
+
+
+def batch(n, seed):
+    rng = np.random.default_rng(seed)
+    X = M.features(10 ** rng.uniform(4, 8, n), rng.choice([7, 8, 18], n), rng.random(n), rng.random(n))
+    return X, rng
+
+
+def test_training_gradients_match_oracle(mp):
+    X, rng = batch(1000, 1)
+    p = rng.normal(0.0, 0.2, M.NPARAM).astype(np.float32)
+    Y = rng.normal(size=(1000, 2)).astype(np.float32)
+    say("device forward/backward, batch 1000")
+    loss, g = LS.debug_grad(mp, p, X, Y, 1.0, 0.5)
+    lref, gref = M.loss_grad(p.astype(np.float64), X.astype(np.float64), Y.astype(np.float64), 1.0, 0.5)
+    say("loss device %.8g oracle %.8g; max |dg| %.3g of max |g| %.3g"
+        % (loss, lref, np.abs(g - gref).max(), np.abs(gref).max()))
+    assert abs(loss - lref) <= 1e-5 * abs(lref)
+    assert np.all(np.abs(g - gref) <= 1e-4 * np.abs(gref).max() + 1e-3 * np.abs(gref))
+
+
+def test_matrix_core_inference_matches_oracle(mp):
+    X, rng = batch(1000, 2)  # not a multiple of 64: the last wave runs partly empty
+    p = rng.normal(0.0, 0.2, M.NPARAM).astype(np.float32)
+    say("matrix-core inference, 1000 rows")
+    y = LS.debug_infer(mp, p, X).astype(np.float64)
+    yb = M.forward_bf16(p, X)
+    y64 = M.forward(p.astype(np.float64), X.astype(np.float64))
+    d = np.abs(y - yb)
+    say("max |y - y_bf16| %.3g mean %.3g; max |y - y_fp64| %.3g (|y| up to %.3g)"
+        % (d.max(), d.mean(), np.abs(y - y64).max(), np.abs(y64).max()))
+    assert d.max() <= 0.02 * (1.0 + np.abs(yb).max())
+    assert d.mean() <= 1e-3 * (1.0 + np.abs(yb).mean())
+    assert np.abs(y - y64).max() <= 0.05 * (1.0 + np.abs(y64).max())
+
+
+SMALL = dict(nenergies=512, samples_per_row=1024, quantiles=128, batch=8192, steps=3000, ks_samples=20000,
+             ks_energies=4)
+
+
+def fresh(seed, n=20000):
+    te, tg, tp = H.tables()
+    m = H.make_device(te, tg, tp, seed, caps=(1 << 20, 1 << 16, 1 << 16), tracked=(True, False, False))
+    m.init_particles(0, n, 1e7 * co.eV)
+    m.init()
+    return m
+
+
+def test_failed_gate_keeps_the_rejection_sampler():
+    a, b = fresh(9), fresh(9)
+    say("training 1 step (must fail the gate)")
+    rep = LS.train(a, LS.SB, **dict(SMALL, steps=1))
+    say(rep)
+    assert rep["installed"] == 0 and rep["ks_max"] > 0.01
+    # bremsstrahlung goes through the sampling kernel, which must still use the rejection
+    # sampler: the particles are those of the plain step
+    a.set_launch(rounds=1, defer_mask=1 << 2)
+    b.set_launch(rounds=0)
+    a.run_steps(0.0, 1e-12, 10, compact_every=4)
+    b.run_steps(0.0, 1e-12, 10, compact_every=4)
+    x, y = H.by_uid(a.download(0)), H.by_uid(b.download(0))
+    assert x.size == y.size and np.array_equal(x["uid"], y["uid"])
+    assert np.array_equal(x["p"], y["p"]) and np.array_equal(x["x"], y["x"])
+    assert a.tally(1)[1] == b.tally(1)[1]
+    a.close()
+    b.close()
+
+
+def test_train_gate_and_gain(mp):
+    say("training the bremsstrahlung sampler (small set)")
+    rep = LS.train(mp, LS.SB, **SMALL)
+    say(rep)
+    assert rep["loss_last"] < 0.05 * rep["loss_first"]
+    assert rep["installed"] == (1 if rep["ks_max"] <= 0.01 else 0)
+    # what training gains: KS of the trained network vs the same network untrained
+    T = np.full(20000, 2e6 * co.eV)
+    exact = LS.sample(mp, LS.SB, False, T, 8, seed=5)
+    trained = LS.sample(mp, LS.SB, True, T, 8, seed=6)
+    p = LS.get_params(mp, LS.SB)
+    rng = np.random.default_rng(0)
+    LS.set_params(mp, LS.SB, rng.normal(0.0, 0.1, p.size).astype(np.float32))
+    untrained = LS.sample(mp, LS.SB, True, T, 8, seed=6)
+    LS.set_params(mp, LS.SB, p)
+
+    def ks(a, b):
+        a, b = np.sort(a), np.sort(b)
+        v = np.concatenate([a, b])
+        return np.abs(np.searchsorted(a, v, "right") / a.size - np.searchsorted(b, v, "right") / b.size).max()
+    kt = max(ks(exact[:, 0], trained[:, 0]), ks(exact[:, 1], trained[:, 1]))
+    ku = max(ks(exact[:, 0], untrained[:, 0]), ks(exact[:, 1], untrained[:, 1]))
+    say("KS at 2 MeV on N: trained %.4f, untrained %.4f" % (kt, ku))
+    assert kt < 0.1 * ku
+
+
+def spectra_ks(a, b):
+    a, b = np.sort(a), np.sort(b)
+    v = np.concatenate([a, b])
+    return np.abs(np.searchsorted(a, v, "right") / a.size - np.searchsorted(b, v, "right") / b.size).max()
+
+
+def test_learned_step_matches_plain_step():
+    """The step with both learned samplers (full training, default configuration) against
+    the plain step on independent streams: electron energy and angle spectra by KS, photon
+    births by count, and the learned samplers' held-out gates."""
+    n, steps = 200_000, 60
+    a = fresh(21, n)
+    t0 = time.time()
+    reps = {w: LS.train(a, w) for w in (LS.SB, LS.MOLLER)}
+    for w, r in reps.items():
+        say("%s: KS max %.4f mean %.4f (noise %.4f), installed %d, train %.1f s"
+            % (LS.NAMES[w], r["ks_max"], r["ks_mean"], r["ks_noise"], r["installed"], r["train_ms"] / 1e3))
+        assert r["installed"] == 1
+    say("training both: %.1f s" % (time.time() - t0))
+    a.set_launch(rounds=1, defer_mask=(1 << 2) | (1 << 4))
+    b = fresh(22, n)
+    for m in (a, b):
+        m.run_steps(0.0, 1e-12, steps, compact_every=8)
+    ea, eb = a.download(0), b.download(0)
+    Ea, Eb = kinenergy(0, ea["p"]), kinenergy(0, eb["p"])
+    ca = -ea["p"][:, 2] / np.linalg.norm(ea["p"], axis=1)
+    cb = -eb["p"][:, 2] / np.linalg.norm(eb["p"], axis=1)
+    kE, kc = spectra_ks(Ea, Eb), spectra_ks(ca, cb)
+    floor = 1.36 * math.sqrt(1 / Ea.size + 1 / Eb.size)
+    na, nb = a.tally(1)[1], b.tally(1)[1]
+    say("electrons %d / %d; KS energy %.4f angle %.4f (noise %.4f); photon births %d / %d"
+        % (Ea.size, Eb.size, kE, kc, floor, na, nb))
+    assert kE <= 0.01 and kc <= 0.01
+    assert abs(na - nb) <= 4 * math.sqrt(na + nb)
+    a.close()
+    b.close()
