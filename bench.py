@@ -37,6 +37,16 @@ WORKLOADS = {
                       name="coupled relativistic avalanche, 1e7 e- log-uniform 0.1-10 MeV, "
                            "dry air 1 atm 273 K, E=1 MV/m, dt=1 ps, e-/photon/e+ tracked, "
                            "cap 2^27, compaction every 8 steps"),
+    "sweep": dict(n=None, e1=1e6, e2=None, loguniform=False, efield=None, dt=1e-12, cap=1 << 28,
+                  compact_every=8, tracked=(1, 0, 0), nswarm=256, count_range=(1000, 1_000_000),
+                  field_range=(0.4e6, 4e6),
+                  name="batched sweep, 256 swarms at E = 0.4-4 MV/m (log-spaced), U{1e3..1e6} e- each "
+                       "at 1 MeV isotropic, dry air 1 atm 273 K, dt=1 ps, photons tallied, per-swarm "
+                       "rate bounds, captured-graph replays with an autotuned launch"),
+    "beam": dict(n=100_000_000, e1=1e7, dt=1e-9, caps=(1 << 23, 1 << 27, 1 << 22), steps=10,
+                 dose=(300, 0.0, 3.0), name="photon pencil beam, 1e8 photons at 10 MeV along +z (1e-6 tilt) into "
+                 "dry air 1 atm 273 K, no field, all secondaries tracked to 10 keV, dt = 1 ns; depth-dose "
+                 "in 300 bins over 0-3 m, photon birth energies in 512 log bins over 10 keV-10 MeV"),
     "swarm": dict(n=10_000, e1=1e6, e2=None, loguniform=False, efield=1e6, dt=1e-12,
                   cap=1 << 20, compact_every=1, tracked=(1, 0, 0),
                   name="runaway swarm, 1e4 e- at 1 MeV, dry air 1 atm 273 K, E=1 MV/m, "
@@ -81,7 +91,8 @@ def cpu_baseline(wl, seed, nsample, nsteps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 300 avalanche, 40 sweep, 10 beam)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="avalanche", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=None, help="override the initial particle count")
@@ -102,6 +113,9 @@ def main():
                     help="only run a plain device-to-device copy of MIB MiB (to scale the HBM "
                          "byte counters of a profiler) and exit")
     args = ap.parse_args()
+    args.steps_given = args.steps is not None
+    if args.steps is None:
+        args.steps = {"sweep": 40, "beam": 10}.get(args.workload, 300)
     if args.copy_calibration:
         import torch
         n = args.copy_calibration * (1 << 20) // 4
@@ -118,7 +132,7 @@ def main():
         sys.exit(2)
 
     traffic = None
-    if not args.no_traffic:
+    if not args.no_traffic and args.workload != "sweep":
         # before this process touches the GPU: the counter passes run as child processes
         try:
             traffic = pmc_traffic(args)
@@ -128,6 +142,10 @@ def main():
     import particulator_hip as P
     from particulator_hip import air, co
 
+    if args.workload == "sweep":
+        return run_sweep(args, traffic)
+    if args.workload == "beam":
+        return run_beam(args)
     wl = dict(WORKLOADS[args.workload])
     n = args.n or wl["n"]
     cut = 1e4 * co.eV
@@ -244,6 +262,129 @@ def pmc_traffic(args):
             "fetch_kib_raw": kib["FETCH_SIZE"][0], "write_kib_raw": kib["WRITE_SIZE"][0],
             "dispatches": kib["FETCH_SIZE"][1], "scale": {"FETCH_SIZE": FETCH_SCALE, "WRITE_SIZE": WRITE_SCALE},
             "note": "counter passes of a 3-step run of the same workload"}
+
+
+def run_beam(args):
+    """The photon pencil beam with its depth-dose and photon-energy tallies."""
+    import numpy as np
+    import particulator_hip as P
+    from particulator_hip import air, co
+    wl = WORKLOADS["beam"]
+    n = args.n or wl["n"]
+    steps = args.steps if args.steps_given else wl["steps"]
+    cut, dt = 1e4 * co.eV, wl["dt"]
+    te, tg, tp = air.build_tables(air.composition(), 1e6, 1e-12, cut)
+    caps = list(wl["caps"])
+    caps[1] = max(caps[1], int(1.1 * n))
+    mp = P.MultiPopulation(seed=args.seed, tally_bins=512, tally_range=(1e4 * co.eV, 1e7 * co.eV),
+                           dose=wl["dose"], tally_tracked=True,
+                           electron=P.Population(caps[0], [], te, cut), photon=P.Population(caps[1], [], tg, cut),
+                           positron=P.Population(caps[2], [], tp, cut))
+    mp.set_pusher(P.RK2Pusher(P.ElectromagneticField(P.HomogeneousField((0.0, 0.0, 0.0)))))
+    mp.init_beam(P.PT_PHOTON, n, wl["e1"] * co.eV)
+    mp.init()
+    t = mp.run_steps(0.0, dt, args.warmup if args.steps_given else 1, compact_every=8, check_every=8)
+    mp.work(reset=True)
+    mp.profile(True)
+    t0 = time.perf_counter()
+    t = mp.run_steps(t, dt, steps, compact_every=8, check_every=steps)
+    el = time.perf_counter() - t0
+    w = mp.work(reset=False)
+    hd, under, over, total = mp.dose()
+    h1, c1, w1 = mp.tally(1)
+    counts = [mp.count(s) for s in range(3)]
+    value = w.particle_steps / el
+    log("beam: %d steps in %.3f s: %.4g particle-steps/s, %.4g collision events/s; populations %s; "
+        "dose %.4g J; photon births %d" % (steps, el, value, w.events / el, counts, total, c1))
+    peak = int(np.argmax(hd))
+    out = {"metric": "particle-steps/s", "value": value, "unit": "particle-steps/s", "n_gpus": 1,
+           "steps": steps, "warmup": 1, "ms_per_step": 1e3 * el / steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (device-generated pencil beam, seeded Philox streams)",
+           "config": {"workload": wl["name"], "initial_particles": n, "seed": args.seed, "population_end": counts,
+                      "collision_events_per_s": w.events / el, "time_s": t},
+           "tallies": {"dose_total_J": total, "dose_under_J": under, "dose_over_J": over,
+                       "dose_peak_bin": peak, "dose_peak_J": float(hd[peak]),
+                       "dose_first_bins_J": [float(v) for v in hd[:4]],
+                       "photon_births": c1, "photon_birth_weight": w1}}
+    print(json.dumps(out))
+    mp.close()
+
+
+def run_sweep(args, traffic):
+    """Independent swarms of uneven sizes at different fields advanced together: one
+    population with per-swarm fields and rate bounds, the launch chosen by the autotuner,
+    steps replayed from a captured graph; then the same with the learned samplers.
+    Per-swarm growth rate and mean energy are reported."""
+    import numpy as np
+    import particulator_hip as P
+    from particulator_hip import air, co, learned as LS
+    wl = WORKLOADS["sweep"]
+    ns = wl["nswarm"]
+    rng = np.random.default_rng(args.seed)
+    counts = rng.integers(wl["count_range"][0], wl["count_range"][1] + 1, size=ns)
+    if args.n:
+        counts = np.maximum(1, (counts * (args.n / counts.sum())).astype(np.int64))
+    fields = wl["field_range"][0] * (wl["field_range"][1] / wl["field_range"][0]) ** (np.arange(ns) / (ns - 1))
+    cut, dt = 1e4 * co.eV, wl["dt"]
+    t_setup = time.perf_counter()
+    te, tg, tp = air.swarm_tables(air.composition(), list(fields), dt, cut)
+    cap = int(min(wl["cap"], max(1 << 20, 2 * counts.sum())))
+    mp = P.MultiPopulation(seed=args.seed, nswarm=ns,
+                           electron=P.Population(cap, [], te, cut, tracked=True),
+                           photon=P.Population(1024, [], tg, cut, tracked=False),
+                           positron=P.Population(1024, [], tp, cut, tracked=False))
+    uid = 0
+    for i in range(ns):
+        mp.set_pusher(P.RK2Pusher(P.ElectromagneticField(P.HomogeneousField((0.0, 0.0, -fields[i])))), swarm=i)
+        mp.init_particles(P.PT_ELECTRON, int(counts[i]), wl["e1"] * co.eV, swarm=i, uid0=uid)
+        uid += int(counts[i])
+    mp.init()
+    s0 = mp.swarm_stats(0)
+    log("sweep setup %.1f s: %d swarms, %d electrons, capacity %d" % (time.perf_counter() - t_setup, ns,
+                                                                         counts.sum(), cap))
+    t = 0.0
+    best, trials, t = mp.autotune(t, dt, trial_steps=1)
+    log("autotuned launch %s (%d trials)" % (best, len(trials)))
+    t = mp.run_graph(t, dt, args.warmup, check_every=max(args.warmup, 1))
+
+    def timed(label):
+        nonlocal t
+        mp.work(reset=True)
+        mp.profile(True)
+        t0 = time.perf_counter()
+        t = mp.run_graph(t, dt, args.steps, check_every=args.steps)
+        el = time.perf_counter() - t0
+        w = mp.work(reset=False)
+        mp.profile(False)
+        log("%s: %d steps in %.3f s: %.4g particle-steps/s" % (label, args.steps, el, w.particle_steps / el))
+        return w.particle_steps / el, el
+    v_plain, el = timed("plain")
+    s1 = mp.swarm_stats(0)
+    # learned samplers, trained on this context's own samplers
+    reps = {LS.NAMES[w]: LS.train(mp, w) for w in (LS.SB, LS.MOLLER)}
+    lb = dict(best, rounds=max(best["rounds"], 1), defer_mask=(1 << 2) | (1 << 4))
+    mp.set_launch(**lb)
+    v_learned, _ = timed("learned")
+    T = t
+    growth = np.log(np.maximum(s1[:, 0], 1) / np.maximum(s0[:, 0], 1)) / max(T - args.steps * dt, dt)
+    emean = s1[:, 2] / np.maximum(s1[:, 1], 1e-300) / co.eV
+    out = {
+        "metric": "particle-steps/s", "value": v_plain, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (device-generated initial particles from a seeded Philox stream)",
+        "config": {"workload": wl["name"], "swarms": ns, "initial_particles": int(counts.sum()),
+                   "seed": args.seed, "launch": best},
+        "learned_batched": {"value": v_learned, "ratio_to_plain": v_learned / v_plain, "launch": lb,
+                            "gates": {k: {"ks_max": r["ks_max"], "installed": r["installed"]} for k, r in reps.items()}},
+        "swarms": {"field_V_per_m": [float(fields[0]), float(fields[ns // 2]), float(fields[-1])],
+                   "growth_rate_per_s": [float(growth[0]), float(growth[ns // 2]), float(growth[-1])],
+                   "mean_energy_eV": [float(emean[0]), float(emean[ns // 2]), float(emean[-1])],
+                   "time_s": T},
+    }
+    print(json.dumps(out))
+    mp.close()
 
 
 def make_world(args, wl, n):

@@ -127,6 +127,9 @@ typedef struct {
     int32_t device;                 /* HIP device ordinal */
     int32_t tally_bins;             /* log-spaced energy bins of the birth tallies */
     double tally_lo, tally_hi;      /* tally energy range (J) */
+    int32_t dose_bins;              /* depth-dose bins in z (0: no dose tally) */
+    int32_t tally_tracked;          /* 1: births of tracked species are tallied too */
+    double dose_zlo, dose_zhi;      /* depth-dose range (m) */
 } pt_config;
 
 /* population statistics (src/population.jl:89-223) */
@@ -188,6 +191,11 @@ int pt_add_particles(pt_ctx* ctx, int species, const double* const soa[PT_NFIELD
  * s = nextcoll(); uid = uid0 + i. */
 int pt_init_particles(pt_ctx* ctx, int species, size_t n, double e1, double e2,
                       int loguniform, int swarm, uint64_t uid0, double t0);
+/* Generate n particles of kinetic energy e at the origin moving along dir (a pencil beam,
+ * the initial photons of scripts/beam.jl:52-60): w = 1, t = t0, s = nextcoll(),
+ * uid = uid0 + i. */
+int pt_init_beam(pt_ctx* ctx, int species, size_t n, double e, const double dir[3], int swarm,
+                 uint64_t uid0, double t0);
 /* Copy a population back (eachparticle, src/population.jl:81).  soa entries may be NULL. */
 int pt_download(pt_ctx* ctx, int species, double* const soa[PT_NFIELDS], uint8_t* active,
                 uint64_t* uid, int32_t* swarm, size_t cap, size_t* n);
@@ -259,6 +267,12 @@ int pt_stats(pt_ctx* ctx, int species, pt_pop_stats* out);
 /* CollisionCounter (src/callback.jl:118-141): out[0] null events, out[1+j] process j in
  * table order; nmax entries at most.  reset != 0 clears the counters afterwards. */
 int pt_collision_counts(pt_ctx* ctx, int species, int64_t* out, int nmax, int reset);
+/* depth-dose tally (build-defined deposition; the reference has none): the kinetic energy
+ * of particles dropped below the cut (droplow!, src/population.jl:273-284) and of
+ * secondaries born below it (add_particle!, src/population.jl:105), and the binding energy
+ * of ionisation (src/rbeb.jl:68) and photoabsorption (src/photo_electric.jl:45), in
+ * weight x J, binned in z; under / over hold what fell outside [dose_zlo, dose_zhi) */
+int pt_dose_read(pt_ctx* ctx, double* hist, int nbins, double* under, double* over, double* total, int reset);
 /* birth tallies of an untracked species: weight histogram over tally bins, count, weight */
 int pt_tally_read(pt_ctx* ctx, int species, double* hist, int nbins, double* count,
                   double* weight);

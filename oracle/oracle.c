@@ -1109,6 +1109,7 @@ int or_collide(const or_proc* pr, int species, const or_state* st, double eng, u
         double phi = TWO_PI * or_rand(g);
         double p1v[3], p2v[3];
         two_body(st, E0, E1, E2, phi, p1v, p2v);
+        out->dep = pr->B;
         out->kind = OR_OUT_NEW;
         new_state(&out->st[0], st->x, p1v, st->w, st->t, st->uid, st->swarm, g);
         new_state(&out->st[1], st->x, p2v, st->w, st->t, 0, st->swarm, g);
@@ -1187,6 +1188,7 @@ int or_collide(const or_proc* pr, int species, const or_state* st, double eng, u
         double pn = or_momentum_norm_from_kin(OR_ELECTRON, ee);
         double p[3];
         or_turn(st->p, cost, phi, pn, p);
+        out->dep = b;
         out->kind = OR_OUT_REPLACE;
         new_state(&out->st[1], st->x, p, st->w, st->t, 0, st->swarm, g);
         out->species[1] = OR_ELECTRON;
@@ -1337,12 +1339,28 @@ void or_setr(or_world* w, int species, or_state* s) {
     s->r = or_ratebound(P->table, sw, i, t);
 }
 
-/* add_particle!, src/population.jl:103-113 (+ untracked species are tallied) */
+/* energy e left at depth z (build-defined dose tally) */
+static void deposit(or_world* w, double e, double z) {
+    if (!w->dose || w->dose_nbins <= 0 || e == 0.0) return;
+    double f = (z - w->dose_zlo) / (w->dose_zhi - w->dose_zlo);
+    int b = f < 0.0 ? w->dose_nbins : (f >= 1.0 ? w->dose_nbins + 1 : (int)floor(f * w->dose_nbins));
+    if (b < 0 || b > w->dose_nbins + 1) b = w->dose_nbins + 1;
+#pragma omp atomic
+    w->dose[b] += e;
+#pragma omp atomic
+    w->dose[w->dose_nbins + 2] += e;
+}
+
+/* add_particle!, src/population.jl:103-113 (+ untracked species are tallied; the energy of
+ * a particle born below the cut is left where it is born) */
 int or_add_particle(or_world* w, int species, const or_state* s) {
     or_pop* P = &w->pop[species];
     double eng = or_kinenergy(species, s->p);
-    if (eng <= P->energy_cut) return -1;
-    if (!P->tracked) {
+    if (eng <= P->energy_cut) {
+        deposit(w, eng * s->w, s->x[2]);
+        return -1;
+    }
+    if (!P->tracked || w->tally_tracked) {
 #pragma omp atomic
         P->tally_n += 1.0;
 #pragma omp atomic
@@ -1355,7 +1373,7 @@ int or_add_particle(or_world* w, int species, const or_state* s) {
                 P->tally_hist[b] += s->w;
             }
         }
-        return -1;
+        if (!P->tracked) return -1;
     }
     size_t slot;
 #pragma omp atomic capture
@@ -1373,6 +1391,7 @@ int or_add_particle(or_world* w, int species, const or_state* s) {
 
 /* apply!, src/collisions.jl:83-133 */
 static void apply_outcome(or_world* w, int species, or_state* st, const or_outcome* o) {
+    deposit(w, o->dep * st->w, st->x[2]);
     switch (o->kind) {
     case OR_OUT_STATECHANGE:
     case OR_OUT_NEW:
@@ -1498,9 +1517,13 @@ void or_repack(or_pop* P) {
 void or_droplow(or_world* w, int species) {
     or_pop* P = &w->pop[species];
     if (!P->parts) return;
-    for (size_t i = 0; i < P->n; i++)
-        if (P->parts[i].active && or_kinenergy(species, P->parts[i].p) < P->energy_cut)
+    for (size_t i = 0; i < P->n; i++) {
+        double e = or_kinenergy(species, P->parts[i].p);
+        if (P->parts[i].active && e < P->energy_cut) {
             P->parts[i].active = 0;
+            deposit(w, e * P->parts[i].w, P->parts[i].x[2]);
+        }
+    }
     or_repack(P);
 }
 

@@ -114,6 +114,7 @@ typedef struct {
     int nnew;
     int species[3];
     or_state st[3];         /* st[0] = new primary state (if kept) */
+    double dep;             /* energy left at the collision point (binding energy) */
 } or_outcome;
 
 /* one population (src/population.jl:7-21) */
@@ -138,6 +139,10 @@ typedef struct {
     int64_t counts[OR_NSPECIES][40]; /* per species: null, then process j+1 */
     int64_t ratebound_violations;
     int64_t passes;
+    /* depth-dose tally (build-defined deposition): kinetic energy of particles dropped
+     * below the cut and of secondaries born below it, binding energies of ionisation and
+     * photoabsorption; dose[nbins] underflow, [nbins + 1] overflow, [nbins + 2] total */
+    double* dose; int dose_nbins; int tally_tracked; double dose_zlo, dose_zhi;
 } or_world;
 
 void or_get_consts(or_consts* out);

@@ -84,7 +84,7 @@ class Field(C.Structure):
 
 class Outcome(C.Structure):
     _fields_ = [("kind", C.c_int), ("nnew", C.c_int), ("species", C.c_int * 3),
-                ("st", State * 3)]
+                ("st", State * 3), ("dep", C.c_double)]
 
 
 class Pop(C.Structure):
@@ -100,7 +100,8 @@ class World(C.Structure):
     _fields_ = [("pop", Pop * 3), ("field", Field * 64), ("nswarm", C.c_int),
                 ("seed", C.c_uint64), ("step", C.c_uint32), ("nthreads", C.c_int),
                 ("counts", (C.c_int64 * 40) * 3), ("ratebound_violations", C.c_int64),
-                ("passes", C.c_int64)]
+                ("passes", C.c_int64), ("dose", C.POINTER(C.c_double)), ("dose_nbins", C.c_int),
+                ("tally_tracked", C.c_int), ("dose_zlo", C.c_double), ("dose_zhi", C.c_double)]
 
 
 _lib = None
@@ -432,7 +433,7 @@ class OracleWorld:
     """MultiPopulation + run loop of the reference (electron, photon, positron order)."""
 
     def __init__(self, tables, seed, caps, cuts, tracked=(True, True, True), nthreads=1,
-                 field=None, fields=None, tally=None):
+                 field=None, fields=None, tally=None, dose=None, tally_tracked=False):
         self.w = World()
         lib().or_world_init(C.byref(self.w), seed, nthreads)
         self.bufs = []
@@ -453,6 +454,13 @@ class OracleWorld:
                 P = self.w.pop[s]
                 P.tally_hist = dptr(h)
                 P.tally_nbins, P.tally_lo, P.tally_hi = nb, lo, hi
+        self.dose = None
+        if dose is not None:
+            nb, zlo, zhi = dose
+            self.dose = np.zeros(nb + 3)
+            self.w.dose = dptr(self.dose)
+            self.w.dose_nbins, self.w.dose_zlo, self.w.dose_zhi = nb, zlo, zhi
+        self.w.tally_tracked = int(tally_tracked)
         if fields is not None:
             for i, f in enumerate(fields):
                 lib().or_set_field(C.byref(self.w), i, C.byref(f))

@@ -100,6 +100,10 @@ struct Params {
     double* tally;  // [3][tally_bins + 2]
     int tally_bins;
     double tally_lo, tally_hi;
+    int tally_tracked;  // births of tracked species are tallied too
+    int dose_bins;      // depth-dose tally: dose[dose_bins] under, [+1] over, [+2] total
+    double* dose;
+    double dose_zlo, dose_zhi;
     ull* range;  // [3][2]
     Clock* clock;
     Queues q;
@@ -206,8 +210,19 @@ __device__ void flush_counters(const Params& P, BlockCounters* bc) {
         if (bc->work[q]) atomicAdd(&P.work[q], bc->work[q]);
 }
 
+// energy e (weight x J) left at depth z: the depth-dose tally
+__device__ __forceinline__ void deposit(const Params& P, double e, double z) {
+    if (P.dose_bins <= 0 || e == 0.0) return;
+    double f = (z - P.dose_zlo) / (P.dose_zhi - P.dose_zlo);
+    int b = f < 0.0 ? P.dose_bins : (f >= 1.0 ? P.dose_bins + 1 : (int)floor(f * P.dose_bins));
+    if (b < 0 || b > P.dose_bins + 1) b = P.dose_bins + 1;
+    atomicAdd(&P.dose[b], e);
+    atomicAdd(&P.dose[P.dose_bins + 2], e);
+}
+
 // ------------------------------------------------------------------ append
-// add_particle!, src/population.jl:103-113 (+ birth tallies of untracked species)
+// add_particle!, src/population.jl:103-113 (+ birth tallies of untracked species; the energy
+// of a particle born below the cut is left where it is born)
 constexpr ull NO_SLOT = ~0ull;
 constexpr int NCHILD = 2;
 
@@ -216,8 +231,11 @@ __device__ __forceinline__ ull append(const Params& P, Shared& sh, int sp2, cons
                                       int swarm, bool main_pass, double tfinal) {
     const SpLite& L = sh.sp[sp2];
     double E = kinenergy(sp2, p);
-    if (E <= L.cut) return NO_SLOT;
-    if (!L.tracked) {
+    if (E <= L.cut) {
+        deposit(P, E * w, x.z);
+        return NO_SLOT;
+    }
+    if (!L.tracked || P.tally_tracked) {
         if (P.tally) {
             double* h = P.tally + (size_t)sp2 * (P.tally_bins + 2);
             atomicAdd(&h[P.tally_bins], 1.0);
@@ -228,7 +246,7 @@ __device__ __forceinline__ ull append(const Params& P, Shared& sh, int sp2, cons
                 if (b >= 0 && b < P.tally_bins) atomicAdd(&h[b], w);
             }
         }
-        return NO_SLOT;
+        if (!L.tracked) return NO_SLOT;
     }
     if (!L.valid) {
         atomicAdd(&sh.bc.flags[FLAG_CAPACITY], 1ull);
@@ -522,6 +540,7 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const u
                                     Ev e;
                                     sample_event(sh.procs[sp][sel], sp, p, pn, E, P.sb, P.pe, g, e);
                                     if (e.err) atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
+                                    if (e.dep != 0.0) deposit(P, e.dep * w, x.z);
                                     V3 m0, m1;
                                     double s0, s1;
                                     finish_event(p, pn, e, g, m0, m1, s0, s1);
@@ -589,7 +608,13 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const u
                 if (nev) atomicAdd(&sh.bc.work[W_EVENTS], (ull)nev);
                 have = false;
             } else if (!(trem > EPS64) || !active) {
-                if (active && kinenergy(sp, p) < L.cut) active = false;  // droplow!
+                if (active) {
+                    double Ek = kinenergy(sp, p);
+                    if (Ek < L.cut) {  // droplow!
+                        active = false;
+                        deposit(P, Ek * w, x.z);
+                    }
+                }
                 L.f[PT_X][idx] = x.x;
                 L.f[PT_Y][idx] = x.y;
                 L.f[PT_Z][idx] = x.z;
@@ -805,6 +830,7 @@ __global__ void __launch_bounds__(BS) k_sample(const Params* __restrict__ Pp) {
             if (use_net) learned_event(sh.procs[sp][sel], sp, E, y0, y1, g, e);
             else sample_event(sh.procs[sp][sel], sp, p, pn, E, P.sb, P.pe, g, e);
             if (e.err) atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
+            if (e.dep != 0.0) deposit(P, e.dep * w, x.z);
             V3 m0, m1;
             double s0, s1;
             finish_event(p, pn, e, g, m0, m1, s0, s1);
@@ -1207,6 +1233,34 @@ __global__ void __launch_bounds__(256) k_init_particles(const Params* __restrict
         S.f[PT_PX][slot] = pn * d.x;
         S.f[PT_PY][slot] = pn * d.y;
         S.f[PT_PZ][slot] = pn * d.z;
+        S.f[PT_W][slot] = 1.0;
+        S.f[PT_T][slot] = t0;
+        S.f[PT_S][slot] = nextcoll(g);
+        S.f[PT_R][slot] = 0.0;
+        S.uid[slot] = uid;
+        S.swarm[slot] = swarm;
+        S.active[slot] = 1;
+    }
+}
+
+// a pencil beam: n particles of kinetic energy e at the origin along the unit vector d
+__global__ void __launch_bounds__(256) k_init_beam(const Params* __restrict__ Pp, int sp, ull base, ull n, double e,
+                                                   double dx, double dy, double dz, int swarm, uint64_t uid0,
+                                                   double t0) {
+    const Params& P = *Pp;
+    const SpeciesDev& S = P.sp[sp];
+    const double pn = mom_from_kin(sp, e);
+    for (ull i = (ull)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ull)gridDim.x * blockDim.x) {
+        ull slot = base + i;
+        uint64_t uid = uid0 + i;
+        PhiloxStream g;
+        g.init(P.seed, uid, 0xFFFFFFFFu);
+        S.f[PT_X][slot] = 0.0;
+        S.f[PT_Y][slot] = 0.0;
+        S.f[PT_Z][slot] = 0.0;
+        S.f[PT_PX][slot] = pn * dx;
+        S.f[PT_PY][slot] = pn * dy;
+        S.f[PT_PZ][slot] = pn * dz;
         S.f[PT_W][slot] = 1.0;
         S.f[PT_T][slot] = t0;
         S.f[PT_S][slot] = nextcoll(g);
@@ -1658,6 +1712,7 @@ struct pt_ctx {
     ull* dwork = nullptr;
     ull* drange = nullptr;
     double* dtally = nullptr;
+    double* ddose = nullptr;
     uint32_t step = 0;
     uint64_t next_uid = 0;
     int grid = 0;         // workgroups of the grid-stride utility kernels
@@ -1880,6 +1935,13 @@ int pt_create(const pt_config* cfg, pt_ctx** out) {
     HIPCHK(c, hipMemsetAsync(c->dtally, 0, sizeof(double) * PT_NSPECIES * (bins + 2), c->stream));
     P.tally = c->dtally;
     P.tally_bins = bins;
+    P.tally_tracked = c->cfg.tally_tracked ? 1 : 0;
+    P.dose_bins = std::max(0, c->cfg.dose_bins);
+    TRY(dalloc(c, (void**)&c->ddose, sizeof(double) * (P.dose_bins + 3)));
+    HIPCHK(c, hipMemsetAsync(c->ddose, 0, sizeof(double) * (P.dose_bins + 3), c->stream));
+    P.dose = c->ddose;
+    P.dose_zlo = c->cfg.dose_zlo;
+    P.dose_zhi = c->cfg.dose_zhi > c->cfg.dose_zlo ? c->cfg.dose_zhi : c->cfg.dose_zlo + 1.0;
     P.tally_lo = c->cfg.tally_lo > 0 ? c->cfg.tally_lo : 1e3 * EV;
     P.tally_hi = c->cfg.tally_hi > P.tally_lo ? c->cfg.tally_hi : 1e7 * EV;
     // fields
@@ -2154,6 +2216,31 @@ int pt_init_particles(pt_ctx* c, int sp, size_t n, double e1, double e2, int log
         int blocks = (int)std::min<size_t>((n + 255) / 256, 65535);
         hipLaunchKernelGGL(k_init_particles, dim3(blocks), dim3(256), 0, c->stream, c->dp, sp,
                            (ull)c->hn[sp], (ull)n, e1, e2, loguniform, swarm, uid0, t0);
+        HIPCHK(c, hipGetLastError());
+    }
+    ull nn = c->hn[sp] + n;
+    HIPCHK(c, hipMemcpyAsync(c->dn + sp, &nn, sizeof(ull), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->hn[sp] = nn;
+    if (uid0 + n > c->next_uid) c->next_uid = uid0 + n;
+    return PT_OK;
+}
+
+int pt_init_beam(pt_ctx* c, int sp, size_t n, double e, const double dir[3], int swarm, uint64_t uid0, double t0) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES || !dir) return fail(c, PT_E_ARG, "bad arguments");
+    if (swarm < 0 || swarm >= c->cfg.nswarm) return fail(c, PT_E_ARG, "bad swarm");
+    double nrm = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    if (!(e > 0) || !(nrm > 0)) return fail(c, PT_E_ARG, "energy and direction must be nonzero");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = read_counts(c);
+    if (rc) return rc;
+    if (c->hn[sp] + n > c->cfg.capacity[sp]) return fail(c, PT_E_CAPACITY, "beam particles exceed capacity");
+    if ((rc = sync_params(c))) return rc;
+    if (n) {
+        int blocks = (int)std::min<size_t>((n + 255) / 256, 65535);
+        hipLaunchKernelGGL(k_init_beam, dim3(blocks), dim3(256), 0, c->stream, c->dp, sp, (ull)c->hn[sp], (ull)n, e,
+                           dir[0] / nrm, dir[1] / nrm, dir[2] / nrm, swarm, uid0, t0);
         HIPCHK(c, hipGetLastError());
     }
     ull nn = c->hn[sp] + n;
@@ -2765,6 +2852,25 @@ int pt_collision_counts(pt_ctx* c, int sp, int64_t* out, int nmax, int reset) {
     for (int q = 0; q < std::min(nmax, NCOUNT); q++) out[q] = (int64_t)h[q];
     if (reset) HIPCHK(c, hipMemsetAsync(c->dcounts + sp * NCOUNT, 0, sizeof(h), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+int pt_dose_read(pt_ctx* c, double* hist, int nbins, double* under, double* over, double* total, int reset) {
+    if (!c) return PT_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int b = c->hp.dose_bins;
+    std::vector<double> h(b + 3);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->ddose, sizeof(double) * (b + 3), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (hist)
+        for (int q = 0; q < std::min(b, nbins); q++) hist[q] = h[q];
+    if (under) *under = h[b];
+    if (over) *over = h[b + 1];
+    if (total) *total = h[b + 2];
+    if (reset) {
+        HIPCHK(c, hipMemsetAsync(c->ddose, 0, sizeof(double) * (b + 3), c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     return PT_OK;
 }
 

@@ -619,6 +619,7 @@ enum { SRC_A = 0, SRC_B = 1, SRC_PMA = 2 };
 
 struct Ev {
     double cA, nA, cB, nB, phi;
+    double dep;  // energy left at the collision point (binding energy)
     int nturn, sgnB, keep, ne;
     int src0, src1, sp0, sp1;
     int err;
@@ -633,6 +634,7 @@ __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3
     e.sgnB = 1;
     e.cB = 0.0;
     e.nB = 0.0;
+    e.dep = 0.0;
     e.src1 = SRC_A;
     e.sp1 = sp;
     switch (pr.kind) {
@@ -657,6 +659,7 @@ __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3
         if (pr.kind == PT_PROC_RBEB) {
             E2 = rbeb_sample(E0, pr.B, g);
             E1 = E0 - E2 - pr.B;
+            e.dep = pr.B;
             if (!(E2 < E1)) e.err = 1;
         } else {
             E2 = pr.kind == PT_PROC_MOLLER ? moller_sample(pr.tcut, eng, g) : bhabha_sample(pr.tcut, eng, g);
@@ -718,6 +721,7 @@ __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3
         }
         if (!(eng > b)) e.err = 3;
         double ee = eng - b;
+        e.dep = b;
         e.cA = sauter_sample(ee, g);
         e.phi = TWO_PI * g.next();
         e.nA = mom_from_kin(PT_ELECTRON, ee);
@@ -779,6 +783,7 @@ __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3
 template <class G>
 __device__ __forceinline__ void learned_event(const DevProc& pr, int sp, double eng, float y0, float y1, G& g, Ev& e) {
     e.err = 0;
+    e.dep = 0.0;
     e.nturn = 1;
     e.sgnB = 1;
     e.cB = 0.0;

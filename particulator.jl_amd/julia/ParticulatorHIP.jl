@@ -49,6 +49,11 @@ end
 struct PtConfig
     capacity::NTuple{3, UInt64}; seed::UInt64; tracked::NTuple{3, Int32}
     nswarm::Int32; device::Int32; tally_bins::Int32; tally_lo::Float64; tally_hi::Float64
+    dose_bins::Int32; tally_tracked::Int32; dose_zlo::Float64; dose_zhi::Float64
+end
+
+struct PtLaunch
+    block::Int32; grid_mult::Int32; compact_every::Int32; rounds::Int32; defer_mask::UInt32
 end
 
 check(ctx, rc) = rc == 0 || error("particulator_hip: ", unsafe_string(
@@ -60,9 +65,9 @@ mutable struct DeviceMultiPopulation
     keep::Vector{Any}
 end
 
-function DeviceMultiPopulation(; capacity=(1 << 20, 1 << 20, 1 << 20), seed=0, tracked=(1, 1, 1))
-    cfg = Ref(PtConfig(UInt64.(capacity), UInt64(seed), Int32.(tracked), 1, 0, 512,
-                       1e4 * Particulator.co.eV, 1e7 * Particulator.co.eV))
+function DeviceMultiPopulation(; capacity=(1 << 20, 1 << 20, 1 << 20), seed=0, tracked=(1, 1, 1), nswarm=1)
+    cfg = Ref(PtConfig(UInt64.(capacity), UInt64(seed), Int32.(tracked), Int32(nswarm), 0, 512,
+                       1e4 * Particulator.co.eV, 1e7 * Particulator.co.eV, 0, 0, 0.0, 0.0))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     rc = ccall((:pt_create, libpt), Cint, (Ref{PtConfig}, Ref{Ptr{Cvoid}}), cfg, h)
     rc == 0 || error("pt_create failed ($rc)")
@@ -119,9 +124,27 @@ fieldfn(f::DoubleLayerField) = PtFieldFn(2, 0, Tuple(f.v), (0.0, 0.0, 0.0), f.z1
 fieldfn(f::StepField) = PtFieldFn(3, 0, Tuple(f.v1), Tuple(f.v2), f.z, 0, 0, 0, 0, 0)
 fieldfn(f::ConfinedDoubleLayerField) = PtFieldFn(4, 0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0, 0, f.sx, f.sy, f.sz, f.ez0)
 
-function set_pusher!(m::DeviceMultiPopulation, pusher::RK2Pusher{<:ElectromagneticField})
+function set_pusher!(m::DeviceMultiPopulation, pusher::RK2Pusher{<:ElectromagneticField}; swarm=0)
     d = Ref(PtFieldDesc(fieldfn(pusher.forcing.e), fieldfn(pusher.forcing.b)))
-    check(m.ctx, ccall((:pt_set_field, libpt), Cint, (Ptr{Cvoid}, Cint, Ref{PtFieldDesc}), m.ctx, 0, d))
+    check(m.ctx, ccall((:pt_set_field, libpt), Cint, (Ptr{Cvoid}, Cint, Ref{PtFieldDesc}), m.ctx, swarm, d))
+end
+
+"""run! with a fixed dt and no host callbacks, replayed from a captured graph."""
+function run_graph!(m::DeviceMultiPopulation, t0, dt, nsteps; check_every=64)
+    check(m.ctx, ccall((:pt_run_graph, libpt), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cint, Cint),
+                       m.ctx, t0, dt, nsteps, check_every))
+    return t0 + nsteps * dt
+end
+
+set_launch!(m::DeviceMultiPopulation, l::PtLaunch) =
+    check(m.ctx, ccall((:pt_set_launch, libpt), Cint, (Ptr{Cvoid}, Ref{PtLaunch}), m.ctx, Ref(l)))
+
+"""count, weight, sum of w*E and max E of each swarm of a species (an nswarm x 4 matrix)."""
+function swarm_stats(m::DeviceMultiPopulation, species, nswarm)
+    out = zeros(Float64, 4, nswarm)
+    check(m.ctx, ccall((:pt_swarm_stats, libpt), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}),
+                       m.ctx, species, nswarm, out))
+    return permutedims(out)
 end
 
 """Append the states of a reference Population (its StructVector columns)."""

@@ -57,7 +57,8 @@ class FieldDesc(C.Structure):
 class Config(C.Structure):
     _fields_ = [("capacity", C.c_uint64 * 3), ("seed", C.c_uint64), ("tracked", C.c_int32 * 3),
                 ("nswarm", C.c_int32), ("device", C.c_int32), ("tally_bins", C.c_int32),
-                ("tally_lo", C.c_double), ("tally_hi", C.c_double)]
+                ("tally_lo", C.c_double), ("tally_hi", C.c_double), ("dose_bins", C.c_int32),
+                ("tally_tracked", C.c_int32), ("dose_zlo", C.c_double), ("dose_zhi", C.c_double)]
 
 
 class PopStats(C.Structure):
@@ -117,6 +118,7 @@ SIGNATURES = {
                                    C.POINTER(C.c_int32), C.c_size_t]),
     "pt_init_particles": (C.c_int, [VP, C.c_int, C.c_size_t, C.c_double, C.c_double, C.c_int,
                                     C.c_int, C.c_uint64, C.c_double]),
+    "pt_init_beam": (C.c_int, [VP, C.c_int, C.c_size_t, C.c_double, DP, C.c_int, C.c_uint64, C.c_double]),
     "pt_download": (C.c_int, [VP, C.c_int, SOA, C.POINTER(C.c_uint8), C.POINTER(C.c_uint64),
                               C.POINTER(C.c_int32), C.c_size_t, C.POINTER(C.c_size_t)]),
     "pt_set_particles": (C.c_int, [VP, C.c_int, SOA, C.POINTER(C.c_uint8),
@@ -143,6 +145,7 @@ SIGNATURES = {
     "pt_debug_mlp_infer": (C.c_int, [VP, FP, FP, C.c_int, FP]),
     "pt_collision_counts": (C.c_int, [VP, C.c_int, C.POINTER(C.c_int64), C.c_int, C.c_int]),
     "pt_tally_read": (C.c_int, [VP, C.c_int, DP, C.c_int, DP, DP]),
+    "pt_dose_read": (C.c_int, [VP, DP, C.c_int, DP, DP, DP, C.c_int]),
     "pt_work_read": (C.c_int, [VP, C.POINTER(Work), C.c_int]),
     "pt_profile_enable": (C.c_int, [VP, C.c_int]),
     "pt_set_step": (C.c_int, [VP, C.c_uint32]),

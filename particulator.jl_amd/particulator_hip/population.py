@@ -161,7 +161,9 @@ class MultiPopulation:
     src/mixed_population.jl:4-18, with all species resident on one GPU."""
 
     def __init__(self, seed=0, device=0, nswarm=1, tally_bins=512, tally_range=(1e4 * co.eV, 1e7 * co.eV),
-                 **pops):
+                 dose=None, tally_tracked=False, **pops):
+        """dose: (bins, zlo, zhi) of a depth-dose tally in z (m); tally_tracked: birth tallies
+        of the tracked species too."""
         self.index = {}
         for name, p in pops.items():
             if name not in SPECIES:
@@ -177,6 +179,9 @@ class MultiPopulation:
         cfg.device = device
         cfg.tally_bins = tally_bins
         cfg.tally_lo, cfg.tally_hi = tally_range
+        if dose is not None:
+            cfg.dose_bins, cfg.dose_zlo, cfg.dose_zhi = dose
+        cfg.tally_tracked = int(bool(tally_tracked))
         self.cfg = cfg
         h = C.c_void_p()
         rc = lib().pt_create(C.byref(cfg), C.byref(h))
@@ -259,6 +264,12 @@ class MultiPopulation:
     def init_particles(self, species, n, e1, e2=None, loguniform=False, swarm=0, uid0=0, t0=0.0):
         check(self.ctx, lib().pt_init_particles(self.ctx, species, n, e1, e2 if e2 else e1,
                                                 int(loguniform), swarm, uid0, t0))
+
+    def init_beam(self, species, n, e, direction=(1e-6, 0.0, 1.0), swarm=0, uid0=0, t0=0.0):
+        """n particles of kinetic energy e (J) at the origin along `direction` (the pencil
+        beam of scripts/beam.jl:52-60)"""
+        d = (C.c_double * 3)(*direction)
+        check(self.ctx, lib().pt_init_beam(self.ctx, species, n, e, d, swarm, uid0, t0))
 
     def count(self, species):
         n = C.c_uint64()
@@ -376,6 +387,15 @@ class MultiPopulation:
         check(self.ctx, lib().pt_tally_read(self.ctx, species, h.ctypes.data_as(C.POINTER(C.c_double)),
                                             nb, C.byref(cnt), C.byref(w)))
         return h[:nb], cnt.value, w.value
+
+    def dose(self, reset=False):
+        """depth-dose tally: (histogram, underflow, overflow, total), weight x J"""
+        nb = self.cfg.dose_bins
+        h = np.zeros(max(nb, 1))
+        u, o, tot = C.c_double(), C.c_double(), C.c_double()
+        check(self.ctx, lib().pt_dose_read(self.ctx, h.ctypes.data_as(C.POINTER(C.c_double)), nb, C.byref(u),
+                                           C.byref(o), C.byref(tot), int(reset)))
+        return h[:nb], u.value, o.value, tot.value
 
     def work(self, reset=False):
         w = Work()
