@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure HBM traffic")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="the learned variant steps from the host instead of replaying a captured graph")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-sampler + captured-graph + autotuned variant")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
@@ -132,7 +134,7 @@ def main():
         sys.exit(2)
 
     traffic = None
-    if not args.no_traffic and args.workload != "sweep":
+    if not args.no_traffic and args.workload == "avalanche":
         # before this process touches the GPU: the counter passes run as child processes
         try:
             traffic = pmc_traffic(args)
@@ -249,13 +251,16 @@ def pmc_traffic(args):
         if not dbs:
             raise RuntimeError("no %s results" % ctr)
         db = sqlite3.connect(dbs[0])
-        row = db.execute("select avg(value), count(*) from counters_collection where counter_name = ? and "
-                         "kernel_name like '%k_advance<%, 0, false>%'", (ctr,)).fetchone()
+        rows = db.execute("select value from counters_collection where counter_name = ? and "
+                          "kernel_name like '%k_advance<%, 0, false, false>%'", (ctr,)).fetchall()
         db.close()
         shutil.rmtree(d, ignore_errors=True)
-        if not row or row[0] is None:
+        vals = [r[0] for r in rows]
+        if not vals:
             raise RuntimeError("no %s samples for the advance kernel" % ctr)
-        kib[ctr] = (row[0], row[1])
+        # the step's main pass (the short passes over overflow secondaries are left out)
+        main = [v for v in vals if v >= 0.1 * max(vals)]
+        kib[ctr] = (sum(main) / len(main), len(main))
     fetch = kib["FETCH_SIZE"][0] * 1024 * FETCH_SCALE
     write = kib["WRITE_SIZE"][0] * 1024 * WRITE_SCALE
     return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
@@ -283,6 +288,8 @@ def run_beam(args):
     mp.set_pusher(P.RK2Pusher(P.ElectromagneticField(P.HomogeneousField((0.0, 0.0, 0.0)))))
     mp.init_beam(P.PT_PHOTON, n, wl["e1"] * co.eV)
     mp.init()
+    # long steps: secondaries of several generations are born in a step
+    mp.set_launch(**dict(mp.get_launch(), tail_passes=6))
     t = mp.run_steps(0.0, dt, args.warmup if args.steps_given else 1, compact_every=8, check_every=8)
     mp.work(reset=True)
     mp.profile(True)
@@ -295,7 +302,10 @@ def run_beam(args):
     counts = [mp.count(s) for s in range(3)]
     value = w.particle_steps / el
     log("beam: %d steps in %.3f s: %.4g particle-steps/s, %.4g collision events/s; populations %s; "
-        "dose %.4g J; photon births %d" % (steps, el, value, w.events / el, counts, total, c1))
+        "dose %.4g J; photon births %d; advance %.1f ms/step, last-workgroup tail %.1f ms/step, secondaries "
+        "advanced in their birth step %d" % (steps, el, value, w.events / el, counts, total, c1,
+                                             w.advance_ms / max(w.advance_launches, 1),
+                                             w.tail_ms / max(w.advance_launches, 1), w.tail_particles))
     peak = int(np.argmax(hd))
     out = {"metric": "particle-steps/s", "value": value, "unit": "particle-steps/s", "n_gpus": 1,
            "steps": steps, "warmup": 1, "ms_per_step": 1e3 * el / steps, "higher_is_better": True,
@@ -424,11 +434,15 @@ def learned_batched(args, wl, n, plain_value):
     best.update(rounds=max(best["rounds"], 1), defer_mask=(1 << 2) | (1 << 4))
     mp.set_launch(**best)
     log("learned samplers trained in %.1f s (installed: %s); autotuned launch %s" % (train_s, installed, best))
-    tnow = mp.run_graph(tnow, dt, args.warmup, check_every=max(args.warmup, 1))
+    def steps(t, k, check):
+        if args.no_graph:
+            return mp.run_steps(t, dt, k, compact_every=best["compact_every"], check_every=check)
+        return mp.run_graph(t, dt, k, check_every=check)
+    tnow = steps(tnow, args.warmup, max(args.warmup, 1))
     mp.work(reset=True)
     mp.profile(True)
     t0 = time.perf_counter()
-    tnow = mp.run_graph(tnow, dt, args.steps, check_every=args.steps)
+    tnow = steps(tnow, args.steps, args.steps)
     el = time.perf_counter() - t0
     w = mp.work(reset=False)
     value = w.particle_steps / el
@@ -436,6 +450,7 @@ def learned_batched(args, wl, n, plain_value):
         % (args.steps, el, value, value / plain_value))
     mp.close()
     return {"value": value, "unit": "particle-steps/s", "ratio_to_plain": value / plain_value,
+            "captured_graph": not args.no_graph,
             "ms_per_step": 1e3 * el / args.steps, "launch": best, "learned_installed": installed,
             "training_s": train_s,
             "gates": {k: {"ks_max": r["ks_max"], "ks_mean": r["ks_mean"], "ks_noise": r["ks_noise"],

@@ -75,6 +75,13 @@ enum pt_field_index {
 
 typedef struct pt_ctx pt_ctx;
 
+/* a state recorded by a wall (WallCallback, src/callback.jl:146-184) */
+typedef struct {
+    double x[3], p[3], w, t;
+    uint64_t uid;
+    int32_t species, swarm;
+} pt_wall_record;
+
 /* One process of a collision table, in table order (the `proc` tuple of
  * ChebyshevCollisionTable, src/collision_table.jl:290-302). */
 typedef struct {
@@ -180,6 +187,13 @@ int pt_upload_pe(pt_ctx* ctx, int slot, int Z, int nshell, const double* binding
 /* pusher forcing ElectromagneticField (src/field.jl:54-70) of one swarm */
 int pt_set_field(pt_ctx* ctx, int swarm, const pt_field_desc* field);
 
+/* ChebContinuumLoss(ContinuumLoss(nel, I, Tcut), Tmax, n) (src/continuum.jl:6-59) added to
+ * the forcing of every swarm as CombinedForcing(field, loss) (src/pusher.jl:14-23):
+ * electron[] and positron[] are the fits' `ec` and `pc` ((order, k+1) column-major, i.e.
+ * [(k+1)][order]) on BinaryIntervals(nintervals, xmax).  NULL arrays remove the loss. */
+int pt_set_continuum_loss(pt_ctx* ctx, int nintervals, int order, double xmax, const double* electron,
+                          const double* positron);
+
 /* ---- particles --------------------------------------------------------------------- */
 /* Append particles (the initial states given to Population(), src/population.jl:34-44).
  * soa[f] points to n values of field f (pt_field_index); uid and swarm may be NULL
@@ -219,6 +233,22 @@ int pt_run(pt_ctx* ctx, double t0, double dt, int nsteps, int compact_every, int
 /* repack! (src/population.jl:229-259): fill holes below the active count with active
  * particles from the end (not order preserving, as in the reference). */
 int pt_compact(pt_ctx* ctx, int species);
+/* ---- device callbacks ---------------------------------------------------------------- */
+/* WallCallback{P}(coord, v, drop) (src/callback.jl:146-184) for one species: a particle
+ * crossing x[coord] = v upwards during a push is recorded as lincomb(new, old, w)
+ * (src/electron.jl:127-132; the positron method's field error, src/positron.jl:51, does
+ * not arise here) and dropped if `drop`.  coord = -1 removes the wall.  Records beyond
+ * `capacity` are counted but not stored.  Needs collision rounds = 0. */
+int pt_set_wall(pt_ctx* ctx, int species, int coord, double v, int drop, size_t capacity);
+/* copy the wall records (n: how many crossings were recorded, possibly > cap) */
+int pt_wall_read(pt_ctx* ctx, pt_wall_record* out, size_t cap, size_t* n, int reset);
+/* roulette!(q, popl) with a constant survival probability (src/population.jl:291-309, the
+ * RouletteCallback of src/callback.jl:210-226): streams keyed by uid and step */
+int pt_roulette(pt_ctx* ctx, int species, double q);
+/* split!(f, popl) with a constant mean (src/population.jl:316-330, the SplitCallback of
+ * src/callback.jl:231-247): weights / (1 + f), Poisson(f) copies with new uids */
+int pt_split(pt_ctx* ctx, int species, double f);
+
 /* ---- launch shape, captured runs, autotuning ---------------------------------------
  * The particle results do not depend on any of these: every random stream is keyed by
  * (seed, particle uid, step), so launch shape and compaction cadence change only where a
@@ -234,6 +264,11 @@ typedef struct {
                                0: every collision is sampled in place */
     uint32_t defer_mask;    /* bit k set: collisions of pt_proc_kind k go through the
                                sampling kernels in collision rounds */
+    int32_t tail_passes;    /* whole-grid passes over the secondaries a step's lanes could
+                               not advance themselves (the advance1! passes,
+                               src/mixed_population.jl:43-46), before one workgroup
+                               finishes the rest (0..16) */
+    int32_t pad;
 } pt_launch;
 /* No reference counterpart (the reference's @batch threads split the loop statically,
  * src/mixed_population.jl:61). */

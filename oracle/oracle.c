@@ -1284,6 +1284,34 @@ static void force(int species, const or_field* f, const double x[3], const doubl
     double cy = v[2] * b[0] - v[0] * b[2];
     double cz = v[0] * b[1] - v[1] * b[0];
     out[0] = q * (e[0] + cx); out[1] = q * (e[1] + cy); out[2] = q * (e[2] + cz);
+    /* CombinedForcing(field, continuum loss): the sum of the forces (src/pusher.jl:14-23) */
+    if (f->cl.on) {
+        double fc = or_chebeval_n(f->cl.c[species == OR_ELECTRON ? 0 : 1], f->cl.k, f->cl.order, f->cl.xmax,
+                                  or_kinenergy(species, p));
+        double sc = -fc / norm3(p);
+        out[0] = out[0] + p[0] * sc; out[1] = out[1] + p[1] * sc; out[2] = out[2] + p[2] * sc;
+    }
+}
+
+/* chebeval with `order` terms (src/cheby.jl:55-81, :111-143): T_j = (2 xi) T_{j-1} - T_{j-2},
+ * summed left to right; the interval index is clamped to [0, k] */
+double or_chebeval_n(const double* c, int k, int order, double xmax, double x) {
+    double t[3];
+    int i = or_precheb(x, k, xmax, t);
+    if (i > k) i = k;
+    if (i < 0) i = 0;
+    const double* a = c + (size_t)i * order;
+    double xi = t[1];
+    double tm2 = 1.0, tm1 = xi;
+    double acc = a[0] * 1.0;
+    if (order > 1) acc = acc + a[1] * xi;
+    for (int j = 2; j < order; j++) {
+        double tj = 2.0 * xi * tm1 - tm2;
+        acc = acc + a[j] * tj;
+        tm2 = tm1;
+        tm1 = tj;
+    }
+    return acc;
 }
 
 /* RK2Pusher advance_particle, src/pusher.jl:41-63 */
@@ -1309,6 +1337,7 @@ void or_push(int species, const or_field* f, or_state* y, double dt) {
 /* ---------------------------------------------------------------- populations */
 int or_world_init(or_world* w, uint64_t seed, int nthreads) {
     memset(w, 0, sizeof(*w));
+    for (int q = 0; q < OR_NSPECIES; q++) w->wall_coord[q] = -1;
     w->seed = seed;
     w->nthreads = nthreads;
     w->nswarm = 1;
@@ -1444,6 +1473,33 @@ static void do_one_collision(or_world* w, int species, or_state* st, or_rng* g) 
     st->s = nextcoll(g);
 }
 
+/* onadvance of WallCallback (src/callback.jl:166-182): a crossing of the wall upwards
+ * records lincomb(new, old, w) (src/electron.jl:127-132), w = (v - old) / (new - old), and
+ * drops the particle if asked */
+static void wall_check(or_world* w, int species, const or_state* old, or_state* nw) {
+    int c = w->wall_coord[species];
+    if (c < 0 || !w->wall_rec) return;
+    double v = w->wall_v[species];
+    if (!(old->x[c] < v && v < nw->x[c])) return;
+    double a = (v - old->x[c]) / (nw->x[c] - old->x[c]);
+    size_t slot;
+#pragma omp atomic capture
+    slot = w->wall_n++;
+    if (slot < w->wall_cap) {
+        or_wallrec* r = &w->wall_rec[slot];
+        for (int d = 0; d < 3; d++) {
+            r->x[d] = nw->x[d] * a + old->x[d] * (1.0 - a);
+            r->p[d] = nw->p[d] * a + old->p[d] * (1.0 - a);
+        }
+        r->w = nw->w * a + old->w * (1.0 - a);
+        r->t = nw->t * a + old->t * (1.0 - a);
+        r->uid = nw->uid;
+        r->species = species;
+        r->swarm = nw->swarm;
+    }
+    if (w->wall_drop[species]) nw->active = 0;
+}
+
 /* body of the @batch loop, src/mixed_population.jl:61-87 */
 static void advance_particle_loop(or_world* w, int species, size_t i, double tfinal) {
     or_pop* P = &w->pop[species];
@@ -1461,7 +1517,9 @@ static void advance_particle_loop(or_world* w, int species, size_t i, double tfi
         int collides;
         if (trem > tnext) { dt = tnext; collides = 1; }
         else { dt = trem; collides = 0; l->s -= dt * l->r; }
+        or_state old = *l;
         or_push(species, f, l, dt);
+        wall_check(w, species, &old, l);
         if (collides && l->active) do_one_collision(w, species, l, &g);
         trem -= dt;
     }

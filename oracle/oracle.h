@@ -105,8 +105,18 @@ typedef struct {
     double sx, sy, sz, ez0; /* confined double layer */
 } or_field_fn;
 
+/* ChebContinuumLoss (src/continuum.jl:21-59): friction -f(E) p/|p| on e- and e+, f a
+ * Chebyshev expansion of `order` terms on BinaryIntervals(k, xmax); coefficients per
+ * species [(k+1)][order] (the reference's (order, k+1) column-major) */
+typedef struct {
+    const double* c[2];  /* electron, positron */
+    double xmax;
+    int k, order, on, pad;
+} or_closs;
+
 typedef struct {
     or_field_fn e, b;
+    or_closs cl;
 } or_field;
 
 typedef struct {
@@ -116,6 +126,13 @@ typedef struct {
     or_state st[3];         /* st[0] = new primary state (if kept) */
     double dep;             /* energy left at the collision point (binding energy) */
 } or_outcome;
+
+/* a state recorded by a wall (WallCallback, src/callback.jl:146-184) */
+typedef struct {
+    double x[3], p[3], w, t;
+    uint64_t uid;
+    int32_t species, swarm;
+} or_wallrec;
 
 /* one population (src/population.jl:7-21) */
 typedef struct {
@@ -143,6 +160,10 @@ typedef struct {
      * below the cut and of secondaries born below it, binding energies of ionisation and
      * photoabsorption; dose[nbins] underflow, [nbins + 1] overflow, [nbins + 2] total */
     double* dose; int dose_nbins; int tally_tracked; double dose_zlo, dose_zhi;
+    /* WallCallback per species (coord -1: none): crossings of x[coord] = v upwards are
+     * recorded (lincomb of the states around the push), and the particle dropped if drop */
+    int wall_coord[OR_NSPECIES]; int wall_drop[OR_NSPECIES]; double wall_v[OR_NSPECIES];
+    or_wallrec* wall_rec; size_t wall_cap, wall_n;
 } or_world;
 
 void or_get_consts(or_consts* out);
@@ -199,6 +220,9 @@ void or_randsphere(or_rng* g, double out[3]);
 /* one collision of process `proc` on a particle in `st` with energy eng (the collide methods of src/) */
 int or_collide(const or_proc* proc, int species, const or_state* st, double eng,
                uint64_t seed, uint32_t step, or_rng* g, or_outcome* out);
+
+/* chebeval of an expansion with `order` terms (src/cheby.jl:111-143) */
+double or_chebeval_n(const double* c, int k, int order, double xmax, double x);
 
 /* pusher (src/pusher.jl:37-63) */
 void or_push(int species, const or_field* f, or_state* s, double dt);

@@ -78,13 +78,27 @@ class FieldFn(C.Structure):
                 ("sy", C.c_double), ("sz", C.c_double), ("ez0", C.c_double)]
 
 
+class Closs(C.Structure):
+    _fields_ = [("c", C.POINTER(C.c_double) * 2), ("xmax", C.c_double), ("k", C.c_int), ("order", C.c_int),
+                ("on", C.c_int), ("pad", C.c_int)]
+
+
 class Field(C.Structure):
-    _fields_ = [("e", FieldFn), ("b", FieldFn)]
+    _fields_ = [("e", FieldFn), ("b", FieldFn), ("cl", Closs)]
 
 
 class Outcome(C.Structure):
     _fields_ = [("kind", C.c_int), ("nnew", C.c_int), ("species", C.c_int * 3),
                 ("st", State * 3), ("dep", C.c_double)]
+
+
+class WallRec(C.Structure):
+    _fields_ = [("x", C.c_double * 3), ("p", C.c_double * 3), ("w", C.c_double), ("t", C.c_double),
+                ("uid", C.c_uint64), ("species", C.c_int32), ("swarm", C.c_int32)]
+
+
+WALL_DTYPE = np.dtype([("x", "<f8", 3), ("p", "<f8", 3), ("w", "<f8"), ("t", "<f8"), ("uid", "<u8"),
+                       ("species", "<i4"), ("swarm", "<i4")])
 
 
 class Pop(C.Structure):
@@ -101,7 +115,9 @@ class World(C.Structure):
                 ("seed", C.c_uint64), ("step", C.c_uint32), ("nthreads", C.c_int),
                 ("counts", (C.c_int64 * 40) * 3), ("ratebound_violations", C.c_int64),
                 ("passes", C.c_int64), ("dose", C.POINTER(C.c_double)), ("dose_nbins", C.c_int),
-                ("tally_tracked", C.c_int), ("dose_zlo", C.c_double), ("dose_zhi", C.c_double)]
+                ("tally_tracked", C.c_int), ("dose_zlo", C.c_double), ("dose_zhi", C.c_double),
+                ("wall_coord", C.c_int * 3), ("wall_drop", C.c_int * 3), ("wall_v", C.c_double * 3),
+                ("wall_rec", C.POINTER(WallRec)), ("wall_cap", C.c_size_t), ("wall_n", C.c_size_t)]
 
 
 _lib = None
@@ -159,6 +175,7 @@ def lib():
             "or_pop_setup": (i, [C.POINTER(World), i, C.POINTER(State), C.c_size_t,
                                  C.POINTER(Table), d, i]),
             "or_set_field": (None, [C.POINTER(World), i, C.POINTER(Field)]),
+            "or_chebeval_n": (d, [DP, i, i, d, d]),
             "or_add_particle": (i, [C.POINTER(World), i, C.POINTER(State)]),
             "or_setr": (None, [C.POINTER(World), i, C.POINTER(State)]),
             "or_advance": (i, [C.POINTER(World), d]),
@@ -415,6 +432,18 @@ def precheb(x, k=32, xmax=None):
     return i, t
 
 
+def with_continuum_loss(f, k, order, xmax, ce, cp):
+    """the field f plus ChebContinuumLoss coefficients ce, cp ([k+1][order] each); the
+    arrays are kept alive on the returned field"""
+    ce = np.ascontiguousarray(ce, dtype=np.float64)
+    cp = np.ascontiguousarray(cp, dtype=np.float64)
+    f.cl.c[0] = dptr(ce)
+    f.cl.c[1] = dptr(cp)
+    f.cl.k, f.cl.order, f.cl.xmax, f.cl.on = k, order, xmax, 1
+    f._keep = (ce, cp)
+    return f
+
+
 def field_homogeneous(e=(0.0, 0.0, 0.0), b=(0.0, 0.0, 0.0)):
     f = Field()
     f.e.kind = FIELD_HOMOGENEOUS
@@ -433,7 +462,8 @@ class OracleWorld:
     """MultiPopulation + run loop of the reference (electron, photon, positron order)."""
 
     def __init__(self, tables, seed, caps, cuts, tracked=(True, True, True), nthreads=1,
-                 field=None, fields=None, tally=None, dose=None, tally_tracked=False):
+                 field=None, fields=None, tally=None, dose=None, tally_tracked=False, walls=None,
+                 wall_cap=1 << 16):
         self.w = World()
         lib().or_world_init(C.byref(self.w), seed, nthreads)
         self.bufs = []
@@ -461,6 +491,13 @@ class OracleWorld:
             self.w.dose = dptr(self.dose)
             self.w.dose_nbins, self.w.dose_zlo, self.w.dose_zhi = nb, zlo, zhi
         self.w.tally_tracked = int(tally_tracked)
+        self.wall_buf = None
+        if walls:
+            self.wall_buf = np.zeros(wall_cap, dtype=WALL_DTYPE)
+            self.w.wall_rec = self.wall_buf.ctypes.data_as(C.POINTER(WallRec))
+            self.w.wall_cap = wall_cap
+            for sp, (coord, v, drop) in walls.items():
+                self.w.wall_coord[sp], self.w.wall_v[sp], self.w.wall_drop[sp] = coord, v, int(drop)
         if fields is not None:
             for i, f in enumerate(fields):
                 lib().or_set_field(C.byref(self.w), i, C.byref(f))
@@ -499,6 +536,10 @@ class OracleWorld:
             self.advance(t + dt, droplow)
             t += dt
         return t
+
+    def wall_records(self):
+        n = min(self.w.wall_n, self.w.wall_cap)
+        return self.wall_buf[:n].copy() if self.wall_buf is not None else np.zeros(0, dtype=WALL_DTYPE)
 
     def particles(self, species, active_only=True):
         n = self.w.pop[species].n

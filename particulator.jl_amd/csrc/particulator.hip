@@ -66,6 +66,7 @@ struct Clock {
     ull cursor;           // work items claimed by the push/select kernel
     ull ecur;             // queued collisions claimed by the sampling kernel
     ull nmark[PT_NSPECIES];  // population counts where the last secondary pass starts
+    uint32_t pass, pad2;     // 0: the step's main pass; > 0: passes over its secondaries
 };
 
 // Work queues of the event rounds: per species two ping-pong queues of particles to
@@ -84,6 +85,18 @@ struct Queues {
 __host__ __device__ constexpr int QWK(int s, int par) { return 2 * s + par; }
 __host__ __device__ constexpr int QEV(int s, int k) { return 2 * PT_NSPECIES + NKQ * s + k; }
 constexpr int NQCNT = 2 * PT_NSPECIES + NKQ * PT_NSPECIES;
+
+// WallCallback (src/callback.jl:146-184) of each species: coord < 0 for none
+struct WallDev {
+    int coord[PT_NSPECIES];
+    int drop[PT_NSPECIES];
+    double v[PT_NSPECIES];
+    pt_wall_record* rec;
+    ull* n;
+    ull cap;
+    int on;
+    int pad;
+};
 
 struct Params {
     SpeciesDev sp[PT_NSPECIES];
@@ -109,6 +122,7 @@ struct Params {
     Queues q;
     uint32_t defer_mask;  // sampler kinds (bit = pt_proc_kind) queued in collision rounds
     MlpDev mlp[NLEARN];   // learned samplers (bremsstrahlung, Moller) used by the sampling kernel
+    WallDev wall;         // WallCallback per species
     int lds_mlp_off;      // byte offset of the staged network in the sampling kernel's LDS
     int lds_rate_off[PT_NSPECIES];
     int lds_rb_off[PT_NSPECIES];
@@ -135,11 +149,14 @@ struct SpLite {
     int tracked, valid, nprocs, k, nswarm, pad;
 };
 
-// block-local counters, flushed once per block
+// block-local counters, flushed once per block (the depth-dose histogram too, when it has
+// at most DOSE_LDS bins: deposits are frequent and would otherwise contend in global memory)
+constexpr int DOSE_LDS = 512;
 struct BlockCounters {
     ull counts[PT_NSPECIES * NCOUNT];
     ull flags[NFLAGS];
     ull work[NWORK];
+    double dose[DOSE_LDS + 2];
 };
 
 struct Shared {
@@ -202,6 +219,9 @@ __device__ __forceinline__ double setr(const SpLite& L, double E, int swarm, boo
 
 __device__ void flush_counters(const Params& P, BlockCounters* bc) {
     __syncthreads();
+    if (P.dose_bins > 0 && P.dose_bins <= DOSE_LDS)
+        for (int q = threadIdx.x; q < P.dose_bins + 2; q += blockDim.x)
+            if (bc->dose[q] != 0.0) atomicAdd(&P.dose[q], bc->dose[q]);
     for (int q = threadIdx.x; q < PT_NSPECIES * NCOUNT; q += blockDim.x)
         if (bc->counts[q]) atomicAdd(&P.counts[q], bc->counts[q]);
     for (int q = threadIdx.x; q < NFLAGS; q += blockDim.x)
@@ -210,14 +230,15 @@ __device__ void flush_counters(const Params& P, BlockCounters* bc) {
         if (bc->work[q]) atomicAdd(&P.work[q], bc->work[q]);
 }
 
-// energy e (weight x J) left at depth z: the depth-dose tally
-__device__ __forceinline__ void deposit(const Params& P, double e, double z) {
+// energy e (weight x J) left at depth z: the depth-dose tally (bins, then under- and
+// overflow; the total is their sum)
+__device__ __forceinline__ void deposit(const Params& P, BlockCounters& bc, double e, double z) {
     if (P.dose_bins <= 0 || e == 0.0) return;
     double f = (z - P.dose_zlo) / (P.dose_zhi - P.dose_zlo);
     int b = f < 0.0 ? P.dose_bins : (f >= 1.0 ? P.dose_bins + 1 : (int)floor(f * P.dose_bins));
     if (b < 0 || b > P.dose_bins + 1) b = P.dose_bins + 1;
-    atomicAdd(&P.dose[b], e);
-    atomicAdd(&P.dose[P.dose_bins + 2], e);
+    if (P.dose_bins <= DOSE_LDS) atomicAdd(&bc.dose[b], e);
+    else atomicAdd(&P.dose[b], e);
 }
 
 // ------------------------------------------------------------------ append
@@ -232,7 +253,7 @@ __device__ __forceinline__ ull append(const Params& P, Shared& sh, int sp2, cons
     const SpLite& L = sh.sp[sp2];
     double E = kinenergy(sp2, p);
     if (E <= L.cut) {
-        deposit(P, E * w, x.z);
+        deposit(P, sh.bc, E * w, x.z);
         return NO_SLOT;
     }
     if (!L.tracked || P.tally_tracked) {
@@ -347,7 +368,38 @@ __device__ __forceinline__ void wave_push(ull* cnt, uint32_t* arr, uint32_t val,
     }
 }
 
-template <int SRC, bool EVENT>
+// onadvance of WallCallback (src/callback.jl:166-182): an upward crossing of x[coord] = v
+// records lincomb(new, old, w) (src/electron.jl:127-132), w = (v - old) / (new - old); the
+// particle is dropped if the wall says so.  The recorded state's s and r are not drawn.
+__device__ __forceinline__ void wall_check(const Params& P, int sp, const V3& xo, const V3& po, double to,
+                                           const V3& x, const V3& p, double w, double t, uint64_t uid,
+                                           int swarm, bool& active) {
+    const int c = P.wall.coord[sp];
+    if (c < 0) return;
+    const double v = P.wall.v[sp];
+    const double a0 = c == 0 ? xo.x : (c == 1 ? xo.y : xo.z);
+    const double a1 = c == 0 ? x.x : (c == 1 ? x.y : x.z);
+    if (!(a0 < v && v < a1)) return;
+    const double a = (v - a0) / (a1 - a0);
+    ull slot = atomicAdd(P.wall.n, 1ull);
+    if (slot < P.wall.cap) {
+        pt_wall_record& r = P.wall.rec[slot];
+        r.x[0] = x.x * a + xo.x * (1.0 - a);
+        r.x[1] = x.y * a + xo.y * (1.0 - a);
+        r.x[2] = x.z * a + xo.z * (1.0 - a);
+        r.p[0] = p.x * a + po.x * (1.0 - a);
+        r.p[1] = p.y * a + po.y * (1.0 - a);
+        r.p[2] = p.z * a + po.z * (1.0 - a);
+        r.w = w * a + w * (1.0 - a);
+        r.t = t * a + to * (1.0 - a);
+        r.uid = uid;
+        r.species = sp;
+        r.swarm = swarm;
+    }
+    if (P.wall.drop[sp]) active = false;
+}
+
+template <int SRC, bool EVENT, bool WALL = false>
 __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const ull* rb, const ull* re,
                                             ull* cursor, double tfinal, uint32_t step, bool main_pass,
                                             int par) {
@@ -506,8 +558,15 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const u
                         coll = false;
                         s -= dt * r;
                     }
-                    push(sp, P.fields[swarm], x, p, t, dt);
-                    if (coll && r != 0.0) {
+                    if constexpr (WALL) {
+                        const V3 xo = x, po = p;
+                        const double to = t;
+                        push(sp, P.fields[swarm], x, p, t, dt);
+                        wall_check(P, sp, xo, po, to, x, p, w, t, uid, swarm, active);
+                    } else {
+                        push(sp, P.fields[swarm], x, p, t, dt);
+                    }
+                    if (coll && r != 0.0 && (!WALL || active)) {
                         double E = kinenergy(sp, p);
                         if (E >= L.cut) {
                             nev++;
@@ -540,7 +599,7 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const u
                                     Ev e;
                                     sample_event(sh.procs[sp][sel], sp, p, pn, E, P.sb, P.pe, g, e);
                                     if (e.err) atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
-                                    if (e.dep != 0.0) deposit(P, e.dep * w, x.z);
+                                    if (e.dep != 0.0) deposit(P, sh.bc, e.dep * w, x.z);
                                     V3 m0, m1;
                                     double s0, s1;
                                     finish_event(p, pn, e, g, m0, m1, s0, s1);
@@ -612,7 +671,7 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const u
                     double Ek = kinenergy(sp, p);
                     if (Ek < L.cut) {  // droplow!
                         active = false;
-                        deposit(P, Ek * w, x.z);
+                        deposit(P, sh.bc, Ek * w, x.z);
                     }
                 }
                 L.f[PT_X][idx] = x.x;
@@ -658,6 +717,26 @@ __global__ void k_snapshot(const Params* __restrict__ Pp, int clocked, double tf
         C.cursor = 0;
         C.ecur = 0;
         C.qpar = 0;
+        C.pass = 0;
+    }
+}
+
+// the next advance1! pass (src/mixed_population.jl:43-46): the particles appended since the
+// last mark, processed by the whole grid
+__global__ void k_next_pass(const Params* __restrict__ Pp) {
+    const Params& P = *Pp;
+    Clock& C = *P.clock;
+    int s = threadIdx.x;
+    if (s < PT_NSPECIES) {
+        ull n = __hip_atomic_load(P.sp[s].n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (n > P.sp[s].cap) n = P.sp[s].cap;
+        P.range[2 * s] = C.nmark[s];
+        P.range[2 * s + 1] = n;
+        C.nmark[s] = n;
+    }
+    if (s == 0) {
+        C.cursor = 0;
+        C.pass += 1;
     }
 }
 
@@ -697,7 +776,7 @@ __global__ void k_rotate(const Params* __restrict__ Pp, int mark) {
 }
 
 // push/select over the step's particles (first round) or the input work queues
-template <int BS, int SRC, bool EVENT>
+template <int BS, int SRC, bool EVENT, bool WALL = false>
 __global__ void __launch_bounds__(BS) k_advance(const Params* __restrict__ Pp) {
     extern __shared__ double lds[];
     __shared__ Shared sh;
@@ -723,7 +802,8 @@ __global__ void __launch_bounds__(BS) k_advance(const Params* __restrict__ Pp) {
     __syncthreads();
     if (skip) return;
     stage(P, lds, sh);
-    select_loop<SRC, EVENT>(P, sh, rb, re, &P.clock->cursor, C.tfinal, C.step, SRC == SRC_RANGE, par);
+    select_loop<SRC, EVENT, WALL>(P, sh, rb, re, &P.clock->cursor, C.tfinal, C.step,
+                                  SRC == SRC_RANGE && C.pass == 0, par);
     flush_counters(P, &sh.bc);
 }
 
@@ -830,7 +910,7 @@ __global__ void __launch_bounds__(BS) k_sample(const Params* __restrict__ Pp) {
             if (use_net) learned_event(sh.procs[sp][sel], sp, E, y0, y1, g, e);
             else sample_event(sh.procs[sp][sel], sp, p, pn, E, P.sb, P.pe, g, e);
             if (e.err) atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
-            if (e.dep != 0.0) deposit(P, e.dep * w, x.z);
+            if (e.dep != 0.0) deposit(P, sh.bc, e.dep * w, x.z);
             V3 m0, m1;
             double s0, s1;
             finish_event(p, pn, e, g, m0, m1, s0, s1);
@@ -879,6 +959,7 @@ __global__ void __launch_bounds__(BS) k_sample(const Params* __restrict__ Pp) {
 // Secondaries born in this step and not yet advanced: one workgroup repeats passes over
 // [begin, n) until a pass appends nothing (the `while n > 0` loop of advance!,
 // src/mixed_population.jl:43-46).
+template <bool WALL>
 __global__ void __launch_bounds__(512) k_tail(const Params* __restrict__ Pp, int maxpasses) {
     extern __shared__ double lds[];
     __shared__ Shared sh;
@@ -912,7 +993,7 @@ __global__ void __launch_bounds__(512) k_tail(const Params* __restrict__ Pp, int
             rb[s] = beg[s];
             re[s] = end[s];
         }
-        select_loop<SRC_RANGE, false>(P, sh, rb, re, &cur, tfinal, step, false, 0);
+        select_loop<SRC_RANGE, false, WALL>(P, sh, rb, re, &cur, tfinal, step, false, 0);
         __threadfence();
         __syncthreads();
         if (threadIdx.x == 0)
@@ -1268,6 +1349,60 @@ __global__ void __launch_bounds__(256) k_init_beam(const Params* __restrict__ Pp
         S.uid[slot] = uid;
         S.swarm[slot] = swarm;
         S.active[slot] = 1;
+    }
+}
+
+// roulette! (src/population.jl:291-309) with a constant survival probability q: a particle
+// survives with probability q and its weight is divided by q.  Stream: key seed ^ ROUL_KEY,
+// counter (uid, step).
+constexpr uint64_t ROUL_KEY = 0x524F554C45545445ull, SPLIT_KEY = 0x53504C4954535054ull;
+__global__ void __launch_bounds__(256) k_roulette(const Params* __restrict__ Pp, int sp, double q, uint32_t step) {
+    const Params& P = *Pp;
+    const SpeciesDev& S = P.sp[sp];
+    ull n = *S.n;
+    if (n > S.cap) n = S.cap;
+    for (ull i = (ull)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ull)gridDim.x * blockDim.x) {
+        if (!S.active[i]) continue;
+        PhiloxStream g;
+        g.init(P.seed ^ ROUL_KEY, S.uid[i], step);
+        if (g.next() < q) S.f[PT_W][i] /= q;
+        else S.active[i] = 0;
+    }
+}
+
+// split! (src/population.jl:316-330) with a constant mean f: every particle's weight is
+// divided by 1 + f and Poisson(f) copies of it are appended (same state, new uids derived
+// from the parent's).  Stream: key seed ^ SPLIT_KEY, counter (uid, step).
+__global__ void __launch_bounds__(256) k_split(const Params* __restrict__ Pp, int sp, double f, uint32_t step,
+                                               ull n0) {
+    const Params& P = *Pp;
+    const SpeciesDev& S = P.sp[sp];
+    const double e0 = exp(-f);
+    for (ull i = (ull)blockIdx.x * blockDim.x + threadIdx.x; i < n0; i += (ull)gridDim.x * blockDim.x) {
+        if (!S.active[i]) continue;
+        const double w = S.f[PT_W][i] / (1.0 + f);
+        S.f[PT_W][i] = w;
+        PhiloxStream g;
+        g.init(P.seed ^ SPLIT_KEY, S.uid[i], step);
+        // Poisson(f) by sequential inversion (Knuth's product of uniforms for small f)
+        int k = 0;
+        double prod = g.next();
+        while (prod > e0 && k < 1000) {
+            k++;
+            prod *= g.next();
+        }
+        for (int c = 0; c < k; c++) {
+            ull slot = atomicAdd(S.n, 1ull);
+            if (slot >= S.cap) {
+                atomicAdd(&P.flags[FLAG_CAPACITY], 1ull);
+                break;
+            }
+            for (int fi = 0; fi < PT_NFIELDS; fi++) S.f[fi][slot] = S.f[fi][i];
+            S.f[PT_W][slot] = w;
+            S.uid[slot] = child_uid(P.seed ^ SPLIT_KEY, S.uid[i], step, (uint32_t)(c >> 1), c & 1);
+            S.swarm[slot] = S.swarm[i];
+            S.active[slot] = 1;
+        }
     }
 }
 
@@ -1677,6 +1812,8 @@ struct Span {
     int kind;
 };
 
+constexpr int MAX_KV = 8;  // kernel variants with a residency entry
+
 struct pt_ctx {
     pt_config cfg{};
     int device = 0;
@@ -1719,8 +1856,8 @@ struct pt_ctx {
     int ncu = 0;          // compute units
     int lds_bytes = 0;
     Clock* dclock = nullptr;
-    pt_launch launch{256, 1, 8, 0, 1u << PT_PROC_SB};
-    int resident[5][3] = {};  // resident workgroups per CU: kernel variant x block 128/256/512
+    pt_launch launch{256, 1, 8, 0, 1u << PT_PROC_SB, 2, 0};
+    int resident[MAX_KV][3] = {};  // resident workgroups per CU: kernel variant x block 128/256/512
     bool queues = false;      // work and collision queues allocated
     int resident_lds = -1;        // LDS size the residency was computed for
     int stat_cap = 0;             // swarms the per-swarm statistics buffer holds
@@ -1736,6 +1873,9 @@ struct pt_ctx {
         double emin = 0.0, emax = 0.0;  // energy range the network is used in (its gated range)
     } learn[NLEARN];
     int sb_z[MAX_SB] = {0, 0, 0, 0};
+    pt_wall_record* dwall = nullptr;  // wall records
+    ull* dwall_n = nullptr;
+    ull wall_cap = 0;
     std::vector<double> sb_le[MAX_SB];  // host copies of the Seltzer-Berger energy grids
     // captured runs: one graph per (steps, compaction) shape
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -1822,7 +1962,8 @@ static int dscratch(pt_ctx* c, Scratch<T>& s, size_t count) {
 
 // kernel variants: history pass over the step's slots, first collision round, later
 // collision rounds, history pass over the work queues, sampling round
-enum { KV_RANGE_FULL = 0, KV_RANGE_EVENT = 1, KV_QUEUE_EVENT = 2, KV_QUEUE_FULL = 3, KV_SAMPLE = 4, NKV = 5 };
+enum { KV_RANGE_FULL = 0, KV_RANGE_EVENT = 1, KV_QUEUE_EVENT = 2, KV_QUEUE_FULL = 3, KV_SAMPLE = 4, KV_RANGE_WALL = 5, NKV = 6 };
+static_assert(NKV <= MAX_KV, "residency table too small for the kernel variants");
 
 template <int BS>
 static const void* kernel_variant(int v) {
@@ -1831,6 +1972,7 @@ static const void* kernel_variant(int v) {
         case KV_RANGE_EVENT: return (const void*)k_advance<BS, SRC_RANGE, true>;
         case KV_QUEUE_EVENT: return (const void*)k_advance<BS, SRC_QUEUE, true>;
         case KV_QUEUE_FULL: return (const void*)k_advance<BS, SRC_QUEUE, false>;
+        case KV_RANGE_WALL: return (const void*)k_advance<BS, SRC_RANGE, false, true>;
         default: return (const void*)k_sample<BS>;
     }
 }
@@ -2421,6 +2563,8 @@ static int launch_step(pt_ctx* c, bool clocked, double tfinal) {
     int rc = ensure_residency(c);
     if (rc) return rc;
     const int R = c->launch.rounds;
+    const bool wall = c->hp.wall.on != 0;
+    if (wall && R > 0) return fail(c, PT_E_STATE, "a wall needs collision rounds = 0");
     if (R > 0 && (rc = ensure_queues(c))) return rc;
     hipLaunchKernelGGL(k_snapshot, dim3(1), dim3(64), 0, c->stream, c->dp, clocked ? 1 : 0, tfinal, c->step);
     HIPCHK(c, hipGetLastError());
@@ -2428,7 +2572,7 @@ static int launch_step(pt_ctx* c, bool clocked, double tfinal) {
     const bool prof = c->profile && !clocked;
     if (prof && (rc = mark(c, &e0))) return rc;
     if (R == 0) {
-        if ((rc = launch_kernel_args(c, KV_RANGE_FULL, clocked))) return rc;
+        if ((rc = launch_kernel_args(c, wall ? KV_RANGE_WALL : KV_RANGE_FULL, clocked))) return rc;
     } else {
         if ((rc = launch_kernel_args(c, KV_RANGE_EVENT, clocked))) return rc;
         for (int r = 1; r <= R; r++) {
@@ -2439,8 +2583,16 @@ static int launch_step(pt_ctx* c, bool clocked, double tfinal) {
         }
         if ((rc = launch_kernel_args(c, KV_QUEUE_FULL, true))) return rc;
     }
+    // secondaries the lanes' stacks could not take: whole-grid passes, then one workgroup
+    // for whatever the last pass created
+    for (int q = 0; q < c->launch.tail_passes; q++) {
+        hipLaunchKernelGGL(k_next_pass, dim3(1), dim3(64), 0, c->stream, c->dp);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = launch_kernel_args(c, wall ? KV_RANGE_WALL : KV_RANGE_FULL, true))) return rc;
+    }
     if (prof && (rc = mark(c, &e1))) return rc;
-    hipLaunchKernelGGL(k_tail, dim3(1), dim3(512), c->lds_bytes, c->stream, c->dp, 100000);
+    if (wall) hipLaunchKernelGGL(k_tail<true>, dim3(1), dim3(512), c->lds_bytes, c->stream, c->dp, 100000);
+    else hipLaunchKernelGGL(k_tail<false>, dim3(1), dim3(512), c->lds_bytes, c->stream, c->dp, 100000);
     HIPCHK(c, hipGetLastError());
     if (prof) {
         if ((rc = mark(c, &e2))) return rc;
@@ -2574,9 +2726,11 @@ static int apply_launch(pt_ctx* c, const pt_launch& l) {
 int pt_set_launch(pt_ctx* c, const pt_launch* l) {
     if (!c) return PT_E_ARG;
     if (!l || (l->block != 128 && l->block != 256 && l->block != 512) || l->grid_mult < 1 ||
-        l->grid_mult > 16 || l->compact_every < 1 || l->rounds < 0 || l->rounds > 8)
+        l->grid_mult > 16 || l->compact_every < 1 || l->rounds < 0 || l->rounds > 8 || l->tail_passes < 0 ||
+        l->tail_passes > 16)
         return fail(c, PT_E_ARG,
-                    "launch: block must be 128, 256 or 512, grid_mult in 1..16, compact_every >= 1, rounds in 0..8");
+                    "launch: block must be 128, 256 or 512, grid_mult in 1..16, compact_every >= 1, rounds in 0..8, "
+                    "tail_passes in 0..16");
     return apply_launch(c, *l);
 }
 
@@ -2588,7 +2742,7 @@ int pt_get_launch(pt_ctx* c, pt_launch* l) {
 
 static bool same_launch(const pt_launch& a, const pt_launch& b) {
     return a.block == b.block && a.grid_mult == b.grid_mult && a.compact_every == b.compact_every &&
-           a.rounds == b.rounds && a.defer_mask == b.defer_mask;
+           a.rounds == b.rounds && a.defer_mask == b.defer_mask && a.tail_passes == b.tail_passes;
 }
 
 // capture `steps` clocked steps (+ one compaction of every species if `compact`)
@@ -2866,7 +3020,11 @@ int pt_dose_read(pt_ctx* c, double* hist, int nbins, double* under, double* over
         for (int q = 0; q < std::min(b, nbins); q++) hist[q] = h[q];
     if (under) *under = h[b];
     if (over) *over = h[b + 1];
-    if (total) *total = h[b + 2];
+    if (total) {
+        double sum = 0.0;
+        for (int q = 0; q < b + 2; q++) sum += h[q];
+        *total = sum;
+    }
     if (reset) {
         HIPCHK(c, hipMemsetAsync(c->ddose, 0, sizeof(double) * (b + 3), c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -3565,6 +3723,123 @@ int pt_debug_mlp_infer(pt_ctx* c, const float* params, const float* X, int n, fl
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(Y, dY.p, sizeof(float) * 2 * (size_t)n, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+
+int pt_set_wall(pt_ctx* c, int sp, int coord, double v, int drop, size_t capacity) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES || coord < -1 || coord > 2) return fail(c, PT_E_ARG, "bad wall arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (coord >= 0 && capacity > c->wall_cap) {
+        int rc;
+        if ((rc = dalloc(c, (void**)&c->dwall, sizeof(pt_wall_record) * capacity))) return rc;
+        if (!c->dwall_n) {
+            if ((rc = dalloc(c, (void**)&c->dwall_n, sizeof(ull)))) return rc;
+            HIPCHK(c, hipMemsetAsync(c->dwall_n, 0, sizeof(ull), c->stream));
+        }
+        c->wall_cap = capacity;
+    }
+    WallDev& W = c->hp.wall;
+    W.coord[sp] = coord;
+    W.v[sp] = v;
+    W.drop[sp] = drop ? 1 : 0;
+    W.rec = c->dwall;
+    W.n = c->dwall_n;
+    W.cap = c->wall_cap;
+    W.on = 0;
+    for (int q = 0; q < PT_NSPECIES; q++)
+        if (W.coord[q] >= 0) W.on = 1;
+    if (W.on && !c->dwall) return fail(c, PT_E_ARG, "a wall needs a record capacity");
+    c->dirty = true;
+    int rc = sync_params(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+int pt_wall_read(pt_ctx* c, pt_wall_record* out, size_t cap, size_t* n, int reset) {
+    if (!c) return PT_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    ull m = 0;
+    if (c->dwall_n) {
+        HIPCHK(c, hipMemcpyAsync(&m, c->dwall_n, sizeof(ull), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (n) *n = (size_t)m;
+    size_t k = (size_t)std::min<ull>(m, c->wall_cap);
+    if (out && k) {
+        if (k > cap) return fail(c, PT_E_CAPACITY, "host buffer of %zu too small for %zu wall records", cap, k);
+        HIPCHK(c, hipMemcpy(out, c->dwall, sizeof(pt_wall_record) * k, hipMemcpyDeviceToHost));
+    }
+    if (reset && c->dwall_n) {
+        HIPCHK(c, hipMemsetAsync(c->dwall_n, 0, sizeof(ull), c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return PT_OK;
+}
+
+int pt_roulette(pt_ctx* c, int sp, double q) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES || !(q > 0.0) || q > 1.0) return fail(c, PT_E_ARG, "bad roulette arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = sync_params(c);
+    if (rc) return rc;
+    if ((rc = read_counts(c))) return rc;
+    if (c->hn[sp]) {
+        int blocks = (int)std::min<ull>((ull)c->grid, (c->hn[sp] + 255) / 256);
+        hipLaunchKernelGGL(k_roulette, dim3(blocks), dim3(256), 0, c->stream, c->dp, sp, q, c->step);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+int pt_split(pt_ctx* c, int sp, double f) {
+    if (!c) return PT_E_ARG;
+    if (sp < 0 || sp >= PT_NSPECIES || !(f >= 0.0) || f > 100.0) return fail(c, PT_E_ARG, "bad split arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = sync_params(c);
+    if (rc) return rc;
+    if ((rc = read_counts(c))) return rc;
+    if (c->hn[sp]) {
+        int blocks = (int)std::min<ull>((ull)c->grid, (c->hn[sp] + 255) / 256);
+        hipLaunchKernelGGL(k_split, dim3(blocks), dim3(256), 0, c->stream, c->dp, sp, f, c->step, (ull)c->hn[sp]);
+        HIPCHK(c, hipGetLastError());
+    }
+    if ((rc = read_counts(c))) return rc;
+    return check_flags(c, nullptr);
+}
+
+
+int pt_set_continuum_loss(pt_ctx* c, int nintervals, int order, double xmax, const double* electron,
+                          const double* positron) {
+    if (!c) return PT_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const bool on = electron && positron;
+    if (on && (nintervals < 1 || nintervals >= MAX_INTERVALS || order < 1 || order > 16 || !(xmax > 0)))
+        return fail(c, PT_E_ARG, "bad continuum-loss table (k in 1..%d, order in 1..16, xmax > 0)", MAX_INTERVALS - 1);
+    ContLoss cl{};
+    if (on) {
+        const size_t m = (size_t)(nintervals + 1) * order;
+        double* d[2] = {nullptr, nullptr};
+        for (int q = 0; q < 2; q++) {
+            int rc = dalloc(c, (void**)&d[q], sizeof(double) * m);
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpyAsync(d[q], q == 0 ? electron : positron, sizeof(double) * m, hipMemcpyHostToDevice,
+                                     c->stream));
+        }
+        cl.c[0] = d[0];
+        cl.c[1] = d[1];
+        cl.xmax = xmax;
+        cl.k = nintervals;
+        cl.order = order;
+        cl.on = 1;
+    }
+    for (auto& F : c->hfields) F.cl = cl;
+    HIPCHK(c, hipMemcpyAsync(c->dfields, c->hfields.data(), sizeof(Field) * c->hfields.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return PT_OK;
 }
 

@@ -72,8 +72,18 @@ struct FieldFn {
     double v[3], v2[3];
     double z1, z2, sx, sy, sz, ez0;
 };
+// ChebContinuumLoss (src/continuum.jl:21-59): friction -f(E) p/|p| on e- and e+, f a
+// Chebyshev expansion of `order` terms on BinaryIntervals(k, xmax); coefficients per species
+// [(k+1)][order] (the reference's (order, k+1) column-major)
+struct ContLoss {
+    const double* c[2];  // electron, positron
+    double xmax;
+    int k, order, on, pad;
+};
+
 struct Field {
     FieldFn e, b;
+    ContLoss cl;  // CombinedForcing(field, continuum loss) when on
 };
 
 // ---------------------------------------------------------------- Philox4x32-10
@@ -575,6 +585,25 @@ __device__ __forceinline__ V3 field_eval(const FieldFn& f, const V3& x) {
 }
 
 // force(::ElectromagneticField, s), src/field.jl:62-68
+// chebeval with `order` terms (src/cheby.jl:55-81, :111-143): T_j = (2 xi) T_{j-1} - T_{j-2},
+// summed left to right; the interval index is clamped to [0, k]
+__device__ __forceinline__ double chebeval_n(const double* c, int k, int order, double xmax, double x) {
+    double t1, t2;
+    bool oob;
+    int i = precheb(x, k, xmax, t1, t2, oob);
+    const double* a = c + (size_t)i * order;
+    double tm2 = 1.0, tm1 = t1;
+    double acc = a[0] * 1.0;
+    if (order > 1) acc = acc + a[1] * t1;
+    for (int j = 2; j < order; j++) {
+        double tj = 2.0 * t1 * tm1 - tm2;
+        acc = acc + a[j] * tj;
+        tm2 = tm1;
+        tm1 = tj;
+    }
+    return acc;
+}
+
 __device__ __forceinline__ V3 force(int sp, const Field& f, const V3& x, const V3& p) {
     if (sp == PT_PHOTON) return V3{0.0, 0.0, 0.0};
     V3 e = field_eval(f.e, x), b = field_eval(f.b, x);
@@ -583,7 +612,15 @@ __device__ __forceinline__ V3 force(int sp, const Field& f, const V3& x, const V
     double cx = v.y * b.z - v.z * b.y;
     double cy = v.z * b.x - v.x * b.z;
     double cz = v.x * b.y - v.y * b.x;
-    return V3{q * (e.x + cx), q * (e.y + cy), q * (e.z + cz)};
+    V3 F{q * (e.x + cx), q * (e.y + cy), q * (e.z + cz)};
+    if (f.cl.on) {
+        // CombinedForcing(field, ContinuumLoss): the sum of the forces (src/pusher.jl:14-23);
+        // the loss term is p * (-f / |p|) (src/continuum.jl:17-20)
+        double fc = chebeval_n(f.cl.c[sp == PT_ELECTRON ? 0 : 1], f.cl.k, f.cl.order, f.cl.xmax, kinenergy(sp, p));
+        double sc = -fc / norm3(p);
+        F = V3{F.x + p.x * sc, F.y + p.y * sc, F.z + p.z * sc};
+    }
+    return F;
 }
 
 // advance_particle(::RK2Pusher, y, dt), src/pusher.jl:41-63

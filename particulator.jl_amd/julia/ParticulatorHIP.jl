@@ -54,6 +54,14 @@ end
 
 struct PtLaunch
     block::Int32; grid_mult::Int32; compact_every::Int32; rounds::Int32; defer_mask::UInt32
+    tail_passes::Int32; pad::Int32
+end
+PtLaunch(; block=256, grid_mult=1, compact_every=8, rounds=0, defer_mask=UInt32(1 << 2), tail_passes=2) =
+    PtLaunch(block, grid_mult, compact_every, rounds, defer_mask, tail_passes, 0)
+
+struct PtWallRecord
+    x::NTuple{3, Float64}; p::NTuple{3, Float64}; w::Float64; t::Float64; uid::UInt64
+    species::Int32; swarm::Int32
 end
 
 check(ctx, rc) = rc == 0 || error("particulator_hip: ", unsafe_string(
@@ -135,6 +143,33 @@ function run_graph!(m::DeviceMultiPopulation, t0, dt, nsteps; check_every=64)
                        m.ctx, t0, dt, nsteps, check_every))
     return t0 + nsteps * dt
 end
+
+"""CombinedForcing(field, ChebContinuumLoss) of every swarm: the fit's `ec`/`pc`
+(src/continuum.jl:21-41) uploaded as they are."""
+function set_continuum_loss!(m::DeviceMultiPopulation, cl)
+    ec, pc = Array{Float64}(cl.ec), Array{Float64}(cl.pc)
+    GC.@preserve ec pc check(m.ctx, ccall((:pt_set_continuum_loss, libpt), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cdouble, Ptr{Float64}, Ptr{Float64}),
+        m.ctx, cl.b.k, size(ec, 1), cl.b.xmax, ec, pc))
+end
+
+"""WallCallback{P}(coord, v, drop) of one species on the device."""
+set_wall!(m::DeviceMultiPopulation, species, coord, v, drop; capacity=1 << 20) =
+    check(m.ctx, ccall((:pt_set_wall, libpt), Cint, (Ptr{Cvoid}, Cint, Cint, Cdouble, Cint, Csize_t),
+                       m.ctx, species, coord - 1, v, drop, capacity))
+
+function wall_records(m::DeviceMultiPopulation; cap=1 << 20, reset=false)
+    out = Vector{PtWallRecord}(undef, cap)
+    n = Ref{Csize_t}(0)
+    check(m.ctx, ccall((:pt_wall_read, libpt), Cint, (Ptr{Cvoid}, Ptr{PtWallRecord}, Csize_t, Ref{Csize_t}, Cint),
+                       m.ctx, out, cap, n, reset))
+    return out[1:min(n[], cap)]
+end
+
+roulette!(m::DeviceMultiPopulation, species, q) =
+    check(m.ctx, ccall((:pt_roulette, libpt), Cint, (Ptr{Cvoid}, Cint, Cdouble), m.ctx, species, q))
+split!(m::DeviceMultiPopulation, species, f) =
+    check(m.ctx, ccall((:pt_split, libpt), Cint, (Ptr{Cvoid}, Cint, Cdouble), m.ctx, species, f))
 
 set_launch!(m::DeviceMultiPopulation, l::PtLaunch) =
     check(m.ctx, ccall((:pt_set_launch, libpt), Cint, (Ptr{Cvoid}, Ref{PtLaunch}), m.ctx, Ref(l)))

@@ -7,16 +7,17 @@ from . import constants
 from . import constants as co
 from ._lib import (PT_ELECTRON, PT_PHOTON, PT_POSITRON, PTError, lib)
 from .air import build_tables, composition
-from .callbacks import (AbstractCallback, CollisionCounter, CombinedCallback,
+from .callbacks import (AbstractCallback, CollisionCounter, CombinedCallback, WallCallback,
                         ParticleCountCallback, PopulationTargetCallback, RouletteCallback,
                         SplitCallback, VoidCallback)
-from .fields import (ConfinedDoubleLayerField, DoubleLayerField, ElectromagneticField,
+from .fields import (CombinedForcing, ConfinedDoubleLayerField, DoubleLayerField, ElectromagneticField,
                      HomogeneousField, NullForcing, RK2Pusher, StepField)
 from .physics import (ORBITALS, RBEB, BetheHeitler, Bhaba, Bhabha, Compton, KleinNishinaCompton,
                       Moller, NullCollision, PhotoElectric, PositronAnihilation,
                       RelativisticCoulomb, SeltzerBerger)
 from .population import (ElectronState, MultiPopulation, PhotonState, Population,
                          PositronState, advance_, droplow_, init_, repack_, roulette_, split_)
+from .continuum import ChebContinuumLoss, ContinuumLoss
 from .run import run_
 from .tables import (BinaryIntervals, ChebyshevCollisionTable, chebfit,
                      collision_table_from_processes)

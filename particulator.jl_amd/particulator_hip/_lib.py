@@ -78,7 +78,8 @@ class Work(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("block", C.c_int32), ("grid_mult", C.c_int32), ("compact_every", C.c_int32),
-                ("rounds", C.c_int32), ("defer_mask", C.c_uint32)]
+                ("rounds", C.c_int32), ("defer_mask", C.c_uint32), ("tail_passes", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 class LearnConfig(C.Structure):
@@ -93,6 +94,11 @@ class LearnReport(C.Structure):
                 ("ks_mean", C.c_double), ("ks_noise", C.c_double), ("gen_ms", C.c_double),
                 ("train_ms", C.c_double), ("eval_ms", C.c_double), ("samples", C.c_int64),
                 ("installed", C.c_int32), ("nparams", C.c_int32)]
+
+
+class WallRecord(C.Structure):
+    _fields_ = [("x", C.c_double * 3), ("p", C.c_double * 3), ("w", C.c_double), ("t", C.c_double),
+                ("uid", C.c_uint64), ("species", C.c_int32), ("swarm", C.c_int32)]
 
 
 class TuneTrial(C.Structure):
@@ -114,6 +120,7 @@ SIGNATURES = {
     "pt_upload_sb": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, C.c_int, DP, DP]),
     "pt_upload_pe": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, DP]),
     "pt_set_field": (C.c_int, [VP, C.c_int, C.POINTER(FieldDesc)]),
+    "pt_set_continuum_loss": (C.c_int, [VP, C.c_int, C.c_int, C.c_double, DP, DP]),
     "pt_add_particles": (C.c_int, [VP, C.c_int, SOA, C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_int32), C.c_size_t]),
     "pt_init_particles": (C.c_int, [VP, C.c_int, C.c_size_t, C.c_double, C.c_double, C.c_int,
@@ -135,6 +142,10 @@ SIGNATURES = {
     "pt_autotune": (C.c_int, [VP, C.c_double, C.c_double, C.c_int, C.POINTER(TuneTrial), C.c_int,
                               C.POINTER(C.c_int), C.POINTER(Launch), DP]),
     "pt_swarm_stats": (C.c_int, [VP, C.c_int, C.c_int, DP]),
+    "pt_set_wall": (C.c_int, [VP, C.c_int, C.c_int, C.c_double, C.c_int, C.c_size_t]),
+    "pt_wall_read": (C.c_int, [VP, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), C.c_int]),
+    "pt_roulette": (C.c_int, [VP, C.c_int, C.c_double]),
+    "pt_split": (C.c_int, [VP, C.c_int, C.c_double]),
     "pt_learn_defaults": (C.c_int, [C.c_int, C.POINTER(LearnConfig)]),
     "pt_learn_train": (C.c_int, [VP, C.c_int, C.POINTER(LearnConfig), C.POINTER(LearnReport)]),
     "pt_learn_enable": (C.c_int, [VP, C.c_int, C.c_int]),

@@ -32,6 +32,11 @@ class CombinedCallback(AbstractCallback):
     def __init__(self, tpl):
         self.tpl = tuple(tpl)
 
+    def install(self, mpopl):
+        for c in self.tpl:
+            if hasattr(c, "install"):
+                c.install(mpopl)
+
     def __getitem__(self, i):
         return self.tpl[i]
 
@@ -65,6 +70,23 @@ class CollisionCounter(AbstractCallback):
                 d[key] = int(c[1 + j])
             out[name] = d
         return out
+
+
+class WallCallback(AbstractCallback):
+    """WallCallback{P}(coord, v, drop), src/callback.jl:146-184, evaluated on the device after
+    every push.  `install(mpopl)` sets the wall (run_ calls it); `accum(mpopl)` returns the
+    recorded states."""
+
+    def __init__(self, species, coord, v, drop=True, capacity=1 << 20):
+        self.species = SPECIES.get(species, species)
+        self.coord, self.v, self.drop, self.capacity = coord, v, drop, capacity
+
+    def install(self, mpopl):
+        mpopl.set_wall(self.species, self.coord, self.v, self.drop, self.capacity)
+
+    def accum(self, mpopl):
+        rec, _ = mpopl.wall_records()
+        return rec[rec["species"] == self.species]
 
 
 class ParticleCountCallback(AbstractCallback):

@@ -104,6 +104,18 @@ class NullForcing:
         return d
 
 
+class CombinedForcing:
+    """CombinedForcing(field, loss), src/pusher.jl:14-23: the electromagnetic field plus a
+    ChebContinuumLoss friction (the device adds the two forces)."""
+
+    def __init__(self, field, continuum=None):
+        self.field = field
+        self.continuum = continuum
+
+    def desc(self):
+        return self.field.desc()
+
+
 class RK2Pusher:
     """Two-stage Runge-Kutta pusher (Ralston weights 1/4, 3/4), src/pusher.jl:37-63."""
 

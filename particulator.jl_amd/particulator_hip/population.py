@@ -243,6 +243,12 @@ class MultiPopulation:
     def set_pusher(self, pusher, swarm=0):
         d = pusher.desc()
         check(self.ctx, lib().pt_set_field(self.ctx, swarm, C.byref(d)))
+        cl = getattr(pusher.forcing, "continuum", None)
+        if cl is not None:
+            self._keep.append(cl)
+            dp = C.POINTER(C.c_double)
+            check(self.ctx, lib().pt_set_continuum_loss(self.ctx, cl.k, cl.order, cl.b.xmax,
+                                                        cl.ec.ctypes.data_as(dp), cl.pc.ctypes.data_as(dp)))
         if swarm == 0:
             self.pusher = pusher
 
@@ -333,17 +339,18 @@ class MultiPopulation:
         check(self.ctx, lib().pt_run_graph(self.ctx, t0, dt, nsteps, check_every))
         return t0 + nsteps * dt
 
-    def set_launch(self, block=256, grid_mult=1, compact_every=8, rounds=0, defer_mask=1 << 2):
-        """Launch shape, compaction cadence of captured runs, collision rounds and the
-        sampler kinds (bit = pt_proc_kind) that collision rounds queue."""
-        l = Launch(block, grid_mult, compact_every, rounds, defer_mask)
+    def set_launch(self, block=256, grid_mult=1, compact_every=8, rounds=0, defer_mask=1 << 2, tail_passes=2):
+        """Launch shape, compaction cadence of captured runs, collision rounds, the sampler
+        kinds (bit = pt_proc_kind) that collision rounds queue, and the whole-grid passes
+        over overflow secondaries."""
+        l = Launch(block, grid_mult, compact_every, rounds, defer_mask, tail_passes, 0)
         check(self.ctx, lib().pt_set_launch(self.ctx, C.byref(l)))
 
     def get_launch(self):
         l = Launch()
         check(self.ctx, lib().pt_get_launch(self.ctx, C.byref(l)))
         return dict(block=l.block, grid_mult=l.grid_mult, compact_every=l.compact_every, rounds=l.rounds,
-                    defer_mask=l.defer_mask)
+                    defer_mask=l.defer_mask, tail_passes=l.tail_passes)
 
     def autotune(self, t0, dt, trial_steps=4):
         """Time the launch shapes and compaction cadences on the live run and keep the
@@ -354,10 +361,11 @@ class MultiPopulation:
                                           C.byref(best), C.byref(tend)))
         out = [dict(block=t.launch.block, grid_mult=t.launch.grid_mult,
                     compact_every=t.launch.compact_every, rounds=t.launch.rounds,
-                    defer_mask=t.launch.defer_mask, rate=t.rate)
+                    defer_mask=t.launch.defer_mask, tail_passes=t.launch.tail_passes, rate=t.rate)
                for t in trials[:min(nt.value, 32)]]
         return (dict(block=best.block, grid_mult=best.grid_mult, compact_every=best.compact_every,
-                     rounds=best.rounds, defer_mask=best.defer_mask), out, tend.value)
+                     rounds=best.rounds, defer_mask=best.defer_mask, tail_passes=best.tail_passes), out,
+                tend.value)
 
     def swarm_stats(self, species, nswarm=None):
         """(nswarm, 4) array: active count, weight, sum of w*E (J), max E (J) per swarm."""
@@ -387,6 +395,26 @@ class MultiPopulation:
         check(self.ctx, lib().pt_tally_read(self.ctx, species, h.ctypes.data_as(C.POINTER(C.c_double)),
                                             nb, C.byref(cnt), C.byref(w)))
         return h[:nb], cnt.value, w.value
+
+    def set_wall(self, species, coord, v, drop=True, capacity=1 << 20):
+        """WallCallback{P}(coord, v, drop) on the device (coord 0..2; -1 removes it)"""
+        check(self.ctx, lib().pt_set_wall(self.ctx, species, coord, v, int(drop), capacity))
+
+    def wall_records(self, reset=False):
+        """the states recorded by the walls (structured array), and the number of crossings"""
+        n = C.c_size_t()
+        check(self.ctx, lib().pt_wall_read(self.ctx, None, 0, C.byref(n), 0))
+        cap = max(n.value, 1)
+        buf = np.zeros(cap, dtype=WALL_DTYPE)
+        n2 = C.c_size_t()
+        check(self.ctx, lib().pt_wall_read(self.ctx, buf.ctypes.data, cap, C.byref(n2), int(reset)))
+        return buf[:min(n.value, cap)], n2.value
+
+    def roulette(self, species, q):
+        check(self.ctx, lib().pt_roulette(self.ctx, species, q))
+
+    def split(self, species, f):
+        check(self.ctx, lib().pt_split(self.ctx, species, f))
 
     def dose(self, reset=False):
         """depth-dose tally: (histogram, underflow, overflow, total), weight x J"""
@@ -447,8 +475,16 @@ def droplow_(popl, thres=0.0):
     popl.mp.compact(popl.species)
 
 
+WALL_DTYPE = np.dtype([("x", "<f8", 3), ("p", "<f8", 3), ("w", "<f8"), ("t", "<f8"), ("uid", "<u8"),
+                       ("species", "<i4"), ("swarm", "<i4")])
+
+
 def roulette_(f, popl, rng=None):
-    """roulette!(f, popl), src/population.jl:291-309 (host-side, on downloaded states)."""
+    """roulette!(f, popl), src/population.jl:291-309.  A constant probability runs on the
+    device (uid-keyed streams); an energy-dependent f runs on the host on downloaded states."""
+    if not callable(f) and rng is None:
+        popl.mp.roulette(popl.species, float(f))
+        return
     g = rng or _host_rng
     arr = popl.mp.download(popl.species, active_only=False)
     act = arr["active"]
@@ -465,7 +501,11 @@ def roulette_(f, popl, rng=None):
 
 
 def split_(f, popl, rng=None):
-    """split!(f, popl), src/population.jl:316-330 (host-side)."""
+    """split!(f, popl), src/population.jl:316-330.  A constant mean runs on the device; a
+    state-dependent f runs on the host."""
+    if not callable(f) and rng is None:
+        popl.mp.split(popl.species, float(f))
+        return
     g = rng or _host_rng
     arr = popl.mp.download(popl.species, active_only=True)
     if arr.size == 0:

@@ -31,6 +31,8 @@ def run_(mpopl, pusher, tfinal, dt, callback=None, output_dt="default", verbosit
         output_dt = tfinal / 20
     if pusher is not mpopl.pusher:
         mpopl.set_pusher(pusher)
+    if hasattr(callback, "install"):
+        callback.install(mpopl)  # device-side parts (walls)
     t = t0
     nxt = t
     isave = 0

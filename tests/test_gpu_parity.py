@@ -222,3 +222,47 @@ def test_collide_injected(dev, tabs, sp):
                 assert int(ouid[i, 1 + q]) == out.st[1 + q].uid
                 assert abs(ost[i, 1 + q, 8] - out.st[1 + q].s) <= 1e-12 * abs(out.st[1 + q].s)
         say("  worst relative difference %.3g" % worst)
+
+
+@pytest.mark.parametrize("sp", [0, 2])
+def test_push_with_continuum_loss_bitexact(tabs, sp):
+    """RK2 push under CombinedForcing(field, ChebContinuumLoss) (src/pusher.jl:14-23,
+    src/continuum.jl:45-59): the device against the oracle, bit for bit."""
+    from particulator_hip.continuum import air_loss
+    te, tg, tp = tabs
+    cheb = air_loss(P.air.composition(), 1e4 * co.eV, n=5)
+    mp = H.make_device(te, tg, tp, seed=7)
+    mp.set_pusher(P.RK2Pusher(P.CombinedForcing(P.ElectromagneticField(P.HomogeneousField((0.0, 0.0, -1e6))),
+                                                cheb)))
+    rng = np.random.default_rng(40 + sp)
+    n = 4096
+    E = np.exp(rng.uniform(np.log(1e4), np.log(1e8), n)) * co.eV
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    pn = np.array([O.lib().or_momentum_norm_from_kin(sp, e) for e in E])
+    soa = np.zeros((10, n))
+    soa[0:3] = rng.normal(size=(3, n))
+    soa[3:6] = (d * pn[:, None]).T
+    soa[7] = rng.uniform(0, 1e-9, n)
+    dt = rng.uniform(0, 1e-12, n)
+    ref = soa.copy()
+    ptrs = SOA(*[soa[f].ctypes.data_as(C.POINTER(C.c_double)) for f in range(10)])
+    say("pt_debug_push with continuum loss, species=%d n=%d" % (sp, n))
+    check(mp.ctx, lib().pt_debug_push(mp.ctx, sp, n, ptrs, dt.ctypes.data_as(C.POINTER(C.c_double))))
+    f = O.with_continuum_loss(O.field_homogeneous((0.0, 0.0, -1e6)), cheb.k, cheb.order, cheb.b.xmax, cheb.ec,
+                              cheb.pc)
+    exact = 0
+    for i in range(n):
+        st = O.State()
+        st.x[:] = list(ref[0:3, i])
+        st.p[:] = list(ref[3:6, i])
+        st.t = ref[7, i]
+        O.lib().or_push(sp, C.byref(f), C.byref(st), dt[i])
+        got = np.concatenate([soa[0:3, i], soa[3:6, i], [soa[7, i]]])
+        want = np.array(list(st.x) + list(st.p) + [st.t])
+        scale = np.array([np.abs(want[:3]).max()] * 3 + [np.abs(want[3:6]).max()] * 3 + [abs(want[6])])
+        assert np.all(np.abs(got - want) <= 1e-12 * scale), i
+        exact += np.array_equal(got, want)
+    say("push with continuum loss bit-exact: %d / %d" % (exact, n))
+    assert exact >= 0.99 * n
+    mp.close()

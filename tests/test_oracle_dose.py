@@ -35,3 +35,37 @@ def test_dose_tally_closes_the_energy_balance(O):
     assert abs(eout - ein) <= 1e-9 * ein
     # the histogram (with under- and overflow) holds the total
     assert abs(w.dose[:-1].sum() - w.dose[-1]) <= 1e-9 * w.dose[-1]
+
+
+def test_wall_records_lie_on_the_wall(O):
+    """WallCallback (src/callback.jl:146-184) in the oracle: photons of a pencil beam cross
+    z = 0.1 m once; the recorded lincomb state lies on the plane at t = z / (c cos theta),
+    with the momentum unchanged (no field), and the particle is dropped."""
+    comp = O.air_composition()
+    cut = 1e4 * co.eV
+    tabs = [O.OracleTable(O.air_processes(s, comp, cut), s, 0.0) for s in (0, 1, 2)]
+    n = 2000
+    st = np.zeros(n, dtype=O.STATE_DTYPE)
+    d = np.array([1e-6, 0.0, 1.0]) / np.sqrt(1.0 + 1e-12)
+    pn = 1e7 * co.eV / co.c
+    st["p"] = pn * d
+    st["w"] = 1.0
+    st["s"] = -np.log(np.random.default_rng(2).random(n))
+    st["active"] = True
+    st["uid"] = np.arange(n, dtype=np.uint64)
+    w = O.OracleWorld(tabs, 3, (1 << 14, 1 << 14, 1 << 14), (cut, cut, cut),
+                      field=O.field_homogeneous((0.0, 0.0, 0.0)), walls={1: (2, 0.1, True)})
+    w.add(1, st)
+    w.init()
+    w.run(0.0, 1e-9, 2)
+    rec = w.wall_records()
+    assert rec.size >= 0.99 * n and rec.size <= n
+    assert np.unique(rec["uid"]).size == rec.size
+    assert np.allclose(rec["x"][:, 2], 0.1, rtol=1e-12, atol=0)
+    # photons that did not scatter before the wall (nearly all: the mean free path is
+    # ~380 m) keep their momentum and reach it at z / (c cos theta)
+    straight = np.all(np.abs(rec["p"] - pn * d) <= 1e-12 * pn, axis=1)
+    assert straight.sum() >= 0.99 * n
+    assert np.allclose(rec["t"][straight], 0.1 / (co.c * d[2]), rtol=1e-12, atol=0)
+    alive = set(w.particles(1)["uid"].tolist())
+    assert not (alive & set(rec["uid"].tolist()))
