@@ -107,6 +107,8 @@ def main():
                     help="skip the two rocprofv3 counter passes that measure HBM traffic")
     ap.add_argument("--no-graph", action="store_true",
                     help="the learned variant steps from the host instead of replaying a captured graph")
+    ap.add_argument("--no-unbatched", action="store_true",
+                    help="sweep: skip the one-swarm-at-a-time plain run")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-sampler + captured-graph + autotuned variant")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
@@ -369,8 +371,10 @@ def run_sweep(args, traffic):
         mp.profile(False)
         log("%s: %d steps in %.3f s: %.4g particle-steps/s" % (label, args.steps, el, w.particle_steps / el))
         return w.particle_steps / el, el
+    t_start = t
     v_plain, el = timed("plain")
     s1 = mp.swarm_stats(0)
+    unb = None if args.no_unbatched else unbatched(args, wl, (te, tg, tp), counts, fields, int(round(t_start / dt)))
     # learned samplers, trained on this context's own samplers
     reps = {LS.NAMES[w]: LS.train(mp, w) for w in (LS.SB, LS.MOLLER)}
     lb = dict(best, rounds=max(best["rounds"], 1), defer_mask=(1 << 2) | (1 << 4))
@@ -387,7 +391,9 @@ def run_sweep(args, traffic):
         "config": {"workload": wl["name"], "swarms": ns, "initial_particles": int(counts.sum()),
                    "seed": args.seed, "launch": best},
         "learned_batched": {"value": v_learned, "ratio_to_plain": v_learned / v_plain, "launch": lb,
+                            "ratio_to_unbatched_plain": v_learned / unb["value"] if unb else None,
                             "gates": {k: {"ks_max": r["ks_max"], "installed": r["installed"]} for k, r in reps.items()}},
+        "unbatched_plain": unb,
         "swarms": {"field_V_per_m": [float(fields[0]), float(fields[ns // 2]), float(fields[-1])],
                    "growth_rate_per_s": [float(growth[0]), float(growth[ns // 2]), float(growth[-1])],
                    "mean_energy_eV": [float(emean[0]), float(emean[ns // 2]), float(emean[-1])],
@@ -395,6 +401,41 @@ def run_sweep(args, traffic):
     }
     print(json.dumps(out))
     mp.close()
+
+
+def unbatched(args, wl, tables, counts, fields, presteps):
+    """The plain GPU form of the sweep: each swarm alone in its own population (one run per
+    field, as the reference runs a sweep), exact samplers, the default launch, steps from
+    the host.  Each swarm first runs `presteps` untimed steps (the batched run's steps
+    before its timed window), then the same timed steps; the figure is the summed
+    particle-steps over the summed timed wall time."""
+    import particulator_hip as P
+    from particulator_hip import co
+    te, tg, tp = tables
+    cut, dt = 1e4 * co.eV, wl["dt"]
+    psteps, el, uid = 0, 0.0, 0
+    for i in range(len(counts)):
+        n = int(counts[i])
+        mp = P.MultiPopulation(seed=args.seed, electron=P.Population(
+            max(1 << 16, 2 * n), [], te.with_ratebounds(te.ratebound[i:i + 1]), cut, tracked=True),
+            photon=P.Population(1024, [], tg, cut, tracked=False),
+            positron=P.Population(1024, [], tp.with_ratebounds(tp.ratebound[i:i + 1]), cut, tracked=False))
+        mp.set_pusher(P.RK2Pusher(P.ElectromagneticField(P.HomogeneousField((0.0, 0.0, -fields[i])))))
+        mp.init_particles(P.PT_ELECTRON, n, wl["e1"] * co.eV, uid0=uid)
+        uid += n
+        mp.init()
+        t = mp.run_steps(0.0, dt, presteps, compact_every=8, check_every=max(presteps, 1))
+        mp.work(reset=True)
+        t0 = time.perf_counter()
+        mp.run_steps(t, dt, args.steps, compact_every=8, check_every=args.steps)
+        el += time.perf_counter() - t0
+        psteps += mp.work(reset=False).particle_steps
+        mp.close()
+    log("unbatched plain: %d swarms one at a time, %d steps each: %.4g particle-steps/s"
+        % (len(counts), args.steps, psteps / el))
+    return {"value": psteps / el, "unit": "particle-steps/s", "seconds": el, "particle_steps": psteps,
+            "untimed_steps_before": presteps,
+            "note": "each swarm in its own population, exact samplers, default launch, host-stepped"}
 
 
 def make_world(args, wl, n):

@@ -172,6 +172,7 @@ int pt_abi_version(void);
  * src/mixed_population.jl:4-12): allocates capacity[s] slots per species. */
 int pt_create(const pt_config* cfg, pt_ctx** out);
 void pt_destroy(pt_ctx* ctx);
+/* the last failure of ctx; with NULL, of the last context-free call on this thread */
 const char* pt_last_error(const pt_ctx* ctx);
 
 /* ---- tables and fields ------------------------------------------------------------ */
@@ -193,6 +194,50 @@ int pt_set_field(pt_ctx* ctx, int swarm, const pt_field_desc* field);
  * [(k+1)][order]) on BinaryIntervals(nintervals, xmax).  NULL arrays remove the loss. */
 int pt_set_continuum_loss(pt_ctx* ctx, int nintervals, int order, double xmax, const double* electron,
                           const double* positron);
+
+/* ---- table building on the host (no context, no device) -----------------------------
+ * The init-time half of the tables: totalcs of every process kind, collision_table_from_
+ * processes (src/collision_table.jl:115-149) with chebfit (src/cheby.jl:211-227),
+ * compratebound (src/collision_table.jl:152-167) and the process order (:137-146), and the
+ * Seltzer-Berger inverse CDFs (src/seltzer.jl:9-64, :163-192).  The outputs are the arrays
+ * pt_upload_table and pt_upload_sb take, so a caller can hand the library raw processes.
+ * On failure pt_last_error(NULL) returns the message (per host thread). */
+typedef struct {
+    int32_t kind;          /* pt_proc_kind */
+    int32_t Z;             /* atomic number of the target */
+    int32_t N;             /* RBEB: electrons in the orbital */
+    int32_t ntab;          /* SB: points of tab_x / tab_y; PHOTOELECTRIC: Sandia intervals */
+    double coef;           /* target density (m^-3): rate = coef * speed * totalcs */
+    double B, U;           /* RBEB: binding and mean kinetic energy of the orbital (J) */
+    double tcut;           /* MOLLER, BHABHA: smallest secondary energy (J) */
+    const double* tab_x;   /* SB: log of the tabulated primary energies (log J);
+                              PHOTOELECTRIC: left edges of the Sandia intervals (J) */
+    const double* tab_y;   /* SB: total cross sections (m^2) at tab_x (pt_sb_build);
+                              PHOTOELECTRIC: ntab rows of 4 coefficients (m^2 J^i, i = 1..4) */
+} pt_process;
+
+/* totalcs(process, E) at n kinetic energies (J), in m^2 (the reference's totalcs methods:
+ * src/relativistic_coulomb.jl:58, src/rbeb.jl:117, src/moller.jl:40, src/bhaba.jl:35,
+ * src/anihilation.jl:25, src/compton.jl:34,59, src/bethe_heitler.jl:28,
+ * src/photo_electric.jl:54, src/seltzer.jl:217) */
+int pt_totalcs(const pt_process* proc, const double* eng, size_t n, double* out);
+/* the pt_proc_desc of a process in a table, with the host-computed constants */
+int pt_process_desc(const pt_process* proc, int slot, pt_proc_desc* out);
+/* SeltzerBerger(Z) from its G4 data file (a RawG4Physics2DVector, src/util.jl:165-190):
+ * nx photon fractions k/T, ny values log(T / energy_scale) and values[ny][nx].  Fills
+ * log_energy[ny] (log J), tcs[ny] (m^2) and data[ny][ncum], the inverse CDF of log(k/T)
+ * at ncum evenly spaced probabilities with k between gamma_min and gamma_max (J). */
+int pt_sb_build(int Z, const double* kfrac, int nx, const double* logT, int ny, const double* values,
+                int ncum, double gamma_min, double gamma_max, double energy_scale,
+                double* log_energy, double* tcs, double* data);
+/* collision_table_from_processes(procs, species, Fdt; safety, nintervals, order,
+ * max_energy) with one rate bound per impulse fdt[s] (Fdt = e E_s dt of swarm s).  Fills
+ * rate[(nintervals+1)][nprocs][order] in table order, ratebound[nswarm][nintervals+1][order],
+ * perm[nprocs] (table position -> index into procs) and keys[nprocs] (sort keys, table
+ * order); any output may be NULL. */
+int pt_table_build(const pt_process* procs, int nprocs, int species, const double* fdt, int nswarm,
+                   double safety, int nintervals, int order, double max_energy, double* rate,
+                   double* ratebound, int32_t* perm, double* keys);
 
 /* ---- particles --------------------------------------------------------------------- */
 /* Append particles (the initial states given to Population(), src/population.jl:34-44).

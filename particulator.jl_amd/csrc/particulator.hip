@@ -30,6 +30,11 @@
 #include "pt_device.h"
 #include "pt_mlp.h"
 
+namespace pt {
+// the message of the last failed context-free call on this host thread (pt_tables.cpp)
+const char* host_last_error();
+}  // namespace pt
+
 using namespace pt;
 
 typedef unsigned long long ull;
@@ -1985,7 +1990,7 @@ extern "C" {
 
 int pt_abi_version(void) { return PT_ABI_VERSION; }
 
-const char* pt_last_error(const pt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* pt_last_error(const pt_ctx* ctx) { return ctx ? ctx->err.c_str() : pt::host_last_error(); }
 
 void pt_destroy(pt_ctx* c) {
     if (!c) return;

@@ -1,0 +1,559 @@
+// pt_tables.cpp -- the init-time half of the reference's collision tables, on the host:
+// total cross sections of every process kind, the Seltzer-Berger inverse CDFs, the
+// per-interval Chebyshev fits, the rate bounds and the process order of
+// collision_table_from_processes (src/collision_table.jl:115-167).  Plain C++ (no device
+// code, no context): a caller hands raw processes in and gets the arrays that
+// pt_upload_table and pt_upload_sb take.  Formulas follow the reference's evaluation order
+// (the library is built with -ffp-contract=off); transcendental functions are the host
+// libm's.
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/particulator_hip.h"
+#include "pt_consts.h"
+
+namespace pt {
+
+namespace {
+
+thread_local std::string g_err;
+
+int host_fail(const char* msg) {
+    g_err = msg;
+    return PT_E_ARG;
+}
+
+constexpr double BARN = 1e-28;
+constexpr int MAX_ORDER = 16;
+constexpr int MAX_K = 63;
+constexpr int SORT_POINTS = 100000;  // LogLinRange(..., 100_000), src/collision_table.jl:139
+
+// ------------------------------------------------------------------ cross sections
+// speed(::Type{Electron|Positron|Photon}, eng), src/electron.jl:47, src/photon.jl:46
+double speed_of(int species, double eng) {
+    if (species == PT_PHOTON) return C;
+    const double q = MC2 / (MC2 + eng);
+    return C * sqrt(1.0 - q * q);
+}
+
+// Julia max(0, v): a NaN stays NaN
+double jmax0(double v) { return isnan(v) ? v : (v > 0.0 ? v : 0.0); }
+
+// totalcs(::RelativisticCoulomb, K), src/relativistic_coulomb.jl:58 (with its 1e-4 eV offset)
+double cs_elastic(double Z, double K) {
+    K = K + 1e-4 * EV;
+    const double a = 1.3413 * pow(Z, -1.0 / 3.0) * A0;
+    const double g = 1.0 + K / (ME * C2);
+    const double p = sqrt(K * (K + 2.0 * (ME * C2))) / C;
+    const double b = p / (g * ME * C);
+    const double al = HBAR * HBAR / (4.0 * (p * p) * (a * a));
+    const double b2 = b * b;
+    return PI * (RE * RE) * (Z * Z) / (pow(b, 4.0) * (g * g)) *
+           ((1.0 + al * b2) / (al * (1.0 + al)) + b2 * log(al / (1.0 + al)));
+}
+
+// totalcs(::RBEB, T), src/rbeb.jl:117
+double cs_rbeb(double B, double U, int N, double T) {
+    if (!(T > B)) return 0.0;
+    const double t1 = T / MC2, b1 = B / MC2, u1 = U / MC2;
+    const double bt2 = 1.0 - 1.0 / ((1.0 + t1) * (1.0 + t1));
+    const double bb2 = 1.0 - 1.0 / ((1.0 + b1) * (1.0 + b1));
+    const double bu2 = 1.0 - 1.0 / ((1.0 + u1) * (1.0 + u1));
+    const double t = T / B;
+    const double pre = 2.0 * (A0 * A0) * PI * pow(ALPHA, 4.0) / (b1 * ((bb2 + bt2) + bu2));
+    const double tt1 = (1.0 + t1) * (1.0 + t1);
+    double v = 1.0 - 1.0 / t;
+    v = v + (b1 * b1 * (t - 1.0)) / (2.0 * tt1);
+    v = v - ((1.0 + 2.0 * t1) * log(t)) / ((1.0 + t) * tt1);
+    v = v - ((t * t - 1.0) * ((bt2 - log(bt2)) + log(2.0 * b1 * (1.0 - bt2)))) / (2.0 * (t * t));
+    return (double)N * (pre * v);
+}
+
+// totalcs(::Moller, eng), src/moller.jl:40
+double cs_moller(double Z, double tcut, double eng) {
+    const double x = tcut / eng;
+    if (x > 1.0) return 0.0;
+    const double g = 1.0 + eng / MC2;
+    const double g2 = g * g;
+    const double b2 = (g2 - 1.0) / g2;
+    double A = ((g - 1.0) * (g - 1.0) / g2) * (0.5 - x) + 1.0 / x;
+    A = A - 1.0 / (1.0 - x);
+    A = A - ((2.0 * g - 1.0) / g2) * log((1.0 - x) / x);
+    return jmax0(TWO_PI * (RE * RE) * Z * A / (g - 1.0) / b2);
+}
+
+// totalcs(::Bhaba, eng), src/bhaba.jl:35
+double cs_bhabha(double Z, double tcut, double eng) {
+    const double g = 1.0 + eng / MC2;
+    const double b = sqrt((g * g - 1.0) / (g * g));
+    const double x = tcut / eng;
+    const double y = 1.0 / (g + 1.0);
+    const double q = 1.0 - 2.0 * y;
+    const double B1 = 2.0 - y * y, B2 = q * (3.0 + y * y), B3 = q * q + q * q * q, B4 = q * q * q;
+    double A = (1.0 / x - 1.0) / (b * b) + B1 * log(x);
+    A = A + B2 * (1.0 - x);
+    A = A - B3 * (1.0 - x * x) / 2.0;
+    A = A + B4 * (1.0 - x * x * x) / 3.0;
+    return jmax0(TWO_PI * (RE * RE) * Z * A / (g - 1.0));
+}
+
+// totalcs(::PositronAnihilation, eng), src/anihilation.jl:25
+double cs_annihilation(double Z, double eng) {
+    const double g = 1.0 + eng / MC2;
+    const double g2 = g * g;
+    const double sq = sqrt(g2 - 1.0);
+    const double A = ((g2 + 4.0 * g + 1.0) / (g2 - 1.0) * log(g + sq) - (g + 3.0) / sq) / (g + 1.0);
+    return Z * PI * (RE * RE) * A;
+}
+
+// totalcs(::KleinNishinaCompton, eng), src/compton.jl:34
+double cs_kn(double Z, double eng) {
+    const double x = 2.0 * eng / MC2;
+    const double s = TWO_PI * (RE * RE) * (1.0 / x) *
+                     ((((1.0 - 4.0 / x) - 8.0 / (x * x)) * log(1.0 + x) + 0.5) + 8.0 / x -
+                      1.0 / ((1.0 + x) * (1.0 + x)) / 2.0);
+    return Z * s;
+}
+
+// totalcs(::Compton, eng), src/compton.jl:59 (Geant4's empirical formula)
+double cs_compton(double Z, double eng) {
+    const double a = 20.0, b = 230.0, c = 440.0;
+    const double d1 = 2.7965e-1 * BARN, d2 = -1.8300e-1 * BARN, d3 = 6.7527 * BARN, d4 = -1.9798e+1 * BARN;
+    const double e1 = 1.9756e-5 * BARN, e2 = -1.0205e-2 * BARN, e3 = -7.3913e-2 * BARN, e4 = 2.7079e-2 * BARN;
+    const double f1 = -3.9178e-7 * BARN, f2 = 6.8241e-5 * BARN, f3 = 6.0480e-5 * BARN, f4 = 3.0274e-4 * BARN;
+    const double Z2 = Z * Z;
+    const double p1Z = Z * ((d1 + e1 * Z) + f1 * Z2);
+    const double p2Z = Z * ((d2 + e2 * Z) + f2 * Z2);
+    const double p3Z = Z * ((d3 + e3 * Z) + f3 * Z2);
+    const double p4Z = Z * ((d4 + e4 * Z) + f4 * Z2);
+    const double T0 = 15.0e3 * EV;
+    auto sig = [&](double X) {
+        return p1Z * log(1.0 + 2.0 * X) / X +
+               ((p2Z + p3Z * X) + p4Z * (X * X)) / (((1.0 + a * X) + b * (X * X)) + c * (X * X * X));
+    };
+    const double X = (eng > T0 ? eng : T0) / MC2;
+    const double sigma = sig(X);
+    if (!(eng < T0)) return sigma;
+    const double X1 = (T0 + 1e3 * EV) / MC2;
+    const double sigma1 = sig(X1);
+    const double c1 = -T0 * (sigma1 - sigma) / (sigma * (1e3 * EV));
+    const double c2 = Z > 1.5 ? 0.375 - 0.0556 * log(Z) : 0.150;
+    const double y = log(eng / T0);
+    return sigma * exp(-y * (c1 + c2 * y));
+}
+
+// totalcs(::BetheHeitler, eng), src/bethe_heitler.jl:28 (Geant4's parametrisation,
+// quadratic below 1.5 MeV)
+double cs_bethe_heitler(double Z, double eng) {
+    if (eng < 2.0 * MC2) return 0.0;
+    const double mb = 1e-34;
+    const double lim = 1.5e6 * EV;
+    static const double A[6] = {8.7842e+2, -1.9625e+3, 1.2949e+3, -2.0028e+2, 1.2575e+1, -2.8333e-1};
+    static const double Bc[6] = {-1.0342e+1, 1.7692e+1, -8.2381, 1.3063, -9.0815e-2, 2.3586e-3};
+    static const double Cc[6] = {-4.5263e+2, 1.1161e+3, -8.6749e+2, 2.1773e+2, -2.0467e+1, 6.5372e-1};
+    const double x = log((eng > lim ? eng : lim) / MC2);
+    double xs[6];
+    xs[1] = x;
+    xs[2] = x * x;
+    xs[3] = xs[2] * x;
+    xs[4] = xs[3] * x;
+    xs[5] = xs[4] * x;
+    auto poly = [&](const double* cf) {
+        double acc = cf[0] * mb + (cf[1] * mb) * xs[1];
+        for (int q = 2; q < 6; q++) acc = acc + (cf[q] * mb) * xs[q];
+        return acc;
+    };
+    const double F1 = poly(A), F2 = poly(Bc), F3 = poly(Cc);
+    double sigma = (Z + 1.0) * ((F1 * Z + F2 * Z * Z) + F3);
+    if (eng < lim) {
+        const double q = (eng - 2.0 * MC2) / (lim - 2.0 * MC2);
+        sigma = sigma * (q * q);
+    }
+    return sigma;
+}
+
+// totalcs(::PhotoElectric, eng), src/photo_electric.jl:54: Sandia parametrisation,
+// sum_i c_i E^-i on the interval whose left edge is the last one <= E
+double cs_photoelectric(const pt_process& p, double eng) {
+    const double* left = p.tab_x;
+    const int n = p.ntab;
+    const int j = (int)(std::upper_bound(left, left + n, eng) - left) - 1;
+    if (j < 0) return 0.0;
+    const double* cf = p.tab_y + 4 * j;
+    const double e1 = 1.0 / eng;
+    const double ep[4] = {e1, e1 * e1, pow(eng, -3.0), pow(eng, -4.0)};
+    double acc = ep[0] * cf[0];
+    for (int i = 1; i < 4; i++) acc = acc + ep[i] * cf[i];
+    return acc;
+}
+
+// totalcs(::SeltzerBerger, K), src/seltzer.jl:217: linear in log E between the tabulated totals (extrapolated
+// linearly from the last pair above the table)
+double cs_sb(const pt_process& p, double K) {
+    const double* le = p.tab_x;
+    const int n = p.ntab;
+    const double lk = log(K);
+    const int i = (int)(std::upper_bound(le, le + n, lk) - le) - 1;
+    if (i < 0) return 0.0;
+    const int ii = std::min(std::max(i, 0), n - 2);
+    const double w = (le[ii + 1] - lk) / (le[ii + 1] - le[ii]);
+    return w * p.tab_y[ii] + (1.0 - w) * p.tab_y[ii + 1];
+}
+
+int check_process(const pt_process& p) {
+    switch (p.kind) {
+    case PT_PROC_ELASTIC:
+    case PT_PROC_MOLLER:
+    case PT_PROC_COMPTON:
+    case PT_PROC_KN_COMPTON:
+    case PT_PROC_BETHE_HEITLER:
+    case PT_PROC_BHABHA:
+    case PT_PROC_ANNIHILATION:
+        if (p.Z < 1) return host_fail("process: Z must be >= 1");
+        return PT_OK;
+    case PT_PROC_RBEB:
+        if (!(p.B > 0.0)) return host_fail("RBEB: B must be > 0");
+        return PT_OK;
+    case PT_PROC_SB:
+        if (p.ntab < 2 || !p.tab_x || !p.tab_y)
+            return host_fail("SeltzerBerger: tab_x/tab_y must hold >= 2 points (pt_sb_build)");
+        return PT_OK;
+    case PT_PROC_PHOTOELECTRIC:
+        if (p.ntab < 1 || !p.tab_x || !p.tab_y)
+            return host_fail("PhotoElectric: tab_x/tab_y must hold the Sandia intervals");
+        return PT_OK;
+    default:
+        return host_fail("unknown process kind");
+    }
+}
+
+double totalcs(const pt_process& p, double eng) {
+    const double Z = (double)p.Z;
+    switch (p.kind) {
+    case PT_PROC_ELASTIC: return cs_elastic(Z, eng);
+    case PT_PROC_RBEB: return cs_rbeb(p.B, p.U, p.N, eng);
+    case PT_PROC_MOLLER: return cs_moller(Z, p.tcut, eng);
+    case PT_PROC_BHABHA: return cs_bhabha(Z, p.tcut, eng);
+    case PT_PROC_ANNIHILATION: return cs_annihilation(Z, eng);
+    case PT_PROC_KN_COMPTON: return cs_kn(Z, eng);
+    case PT_PROC_COMPTON: return cs_compton(Z, eng);
+    case PT_PROC_BETHE_HEITLER: return cs_bethe_heitler(Z, eng);
+    case PT_PROC_PHOTOELECTRIC: return cs_photoelectric(p, eng);
+    case PT_PROC_SB: return cs_sb(p, eng);
+    default: return 0.0;
+    }
+}
+
+// ------------------------------------------------------------------ Chebyshev fits
+// BinaryIntervals(k, xmax), src/cheby.jl:15-36: interval 0 is (0, 2^-k xmax), interval
+// i >= 1 is [2^(i-k-1), 2^(i-k)) xmax
+struct Intervals {
+    int k;
+    double xmax;
+    void bounds(int i, double* lo, double* hi) const {
+        if (i == 0) {
+            double l1, h1;
+            bounds(1, &l1, &h1);
+            *lo = 0.0;
+            *hi = l1;
+            return;
+        }
+        const double l = ldexp(1.0, i - k - 1);
+        *lo = l * xmax;
+        *hi = 2.0 * l * xmax;
+    }
+    // precheb, src/cheby.jl:127-143: interval index (clamped to k) and xi in [-1, 1)
+    int locate(double x, double* xi) const {
+        const double x1 = x / xmax;
+        int l = 0;
+        const double s = frexp(x1, &l);
+        int i = x1 == 0.0 ? 0 : l + k;
+        if (i > 0)
+            *xi = 4.0 * s - 3.0;
+        else {
+            *xi = ldexp(1.0, k + 1) * x1 - 1.0;
+            i = 0;
+        }
+        return i < k ? i : k;
+    }
+};
+
+// chebnodes(n, a, b), src/cheby.jl:41-43
+void nodes(int n, double a, double b, double* x) {
+    for (int q = 0; q < n; q++) x[q] = (a + b) / 2.0 + (b - a) / 2.0 * cos((2.0 * q + 1.0) * PI / (2.0 * n));
+}
+
+// T_0..T_{n-1} at xi by the three-term recurrence
+void chebvals(double xi, int n, double* t) {
+    t[0] = 1.0;
+    if (n > 1) t[1] = xi;
+    for (int j = 2; j < n; j++) t[j] = 2.0 * xi * t[j - 1] - t[j - 2];
+}
+
+// A \ fx by LU with partial pivoting (src/cheby.jl:222): A is n x n row-major, b is
+// overwritten with the solution
+bool lu_solve(double* A, double* b, int n) {
+    int piv[MAX_ORDER];
+    for (int c = 0; c < n; c++) {
+        int p = c;
+        double big = fabs(A[c * n + c]);
+        for (int r = c + 1; r < n; r++)
+            if (fabs(A[r * n + c]) > big) {
+                big = fabs(A[r * n + c]);
+                p = r;
+            }
+        piv[c] = p;
+        if (big == 0.0) return false;
+        if (p != c)
+            for (int q = 0; q < n; q++) std::swap(A[c * n + q], A[p * n + q]);
+        const double inv = 1.0 / A[c * n + c];
+        for (int r = c + 1; r < n; r++) A[r * n + c] *= inv;
+        for (int r = c + 1; r < n; r++)
+            for (int q = c + 1; q < n; q++) A[r * n + q] -= A[r * n + c] * A[c * n + q];
+    }
+    for (int c = 0; c < n; c++)
+        if (piv[c] != c) std::swap(b[c], b[piv[c]]);
+    for (int r = 1; r < n; r++)
+        for (int q = 0; q < r; q++) b[r] -= A[r * n + q] * b[q];
+    for (int r = n - 1; r >= 0; r--) {
+        for (int q = r + 1; q < n; q++) b[r] -= A[r * n + q] * b[q];
+        b[r] /= A[r * n + r];
+    }
+    return true;
+}
+
+// chebfit(f, b, n), src/cheby.jl:211-227: coefficients out[(k+1)][n]
+template <class F>
+bool chebfit(const Intervals& iv, int n, F&& f, double* out) {
+    double x[MAX_ORDER], A[MAX_ORDER * MAX_ORDER], t[MAX_ORDER];
+    for (int i = 0; i <= iv.k; i++) {
+        double lo, hi;
+        iv.bounds(i, &lo, &hi);
+        nodes(n, lo, hi, x);
+        double* c = out + (size_t)i * n;
+        for (int q = 0; q < n; q++) {
+            c[q] = f(x[q]);
+            const double xi = (2.0 * x[q] - (lo + hi)) / (hi - lo);
+            chebvals(xi, n, t);
+            for (int j = 0; j < n; j++) A[q * n + j] = t[j];
+        }
+        if (!lu_solve(A, c, n)) return false;
+    }
+    return true;
+}
+
+// the expansion and its derivative (src/cheby.jl:111-119, :180-203) on one interval
+double cheb_value(const double* a, int n, double xi) {
+    double t[MAX_ORDER];
+    chebvals(xi, n, t);
+    double acc = a[0] * t[0];
+    for (int o = 1; o < n; o++) acc = acc + a[o] * t[o];
+    return acc;
+}
+
+double cheb_slope(const double* a, int n, double xi, double lo, double hi) {
+    double u0 = 1.0, u1 = 2.0 * xi;
+    double df = n > 1 ? a[1] : 0.0;
+    for (int q = 3; q <= n; q++) {
+        df = df + (q - 1) * u1 * a[q - 1];
+        const double u2 = 2.0 * xi * u1 - u0;
+        u0 = u1;
+        u1 = u2;
+    }
+    return 2.0 * df / (hi - lo);
+}
+
+}  // namespace
+
+const char* host_last_error() { return g_err.empty() ? "null context" : g_err.c_str(); }
+
+}  // namespace pt
+
+using namespace pt;
+
+extern "C" {
+
+int pt_totalcs(const pt_process* proc, const double* eng, size_t n, double* out) {
+    if (!proc || (n && (!eng || !out))) return host_fail("pt_totalcs: null argument");
+    int rc = check_process(*proc);
+    if (rc) return rc;
+    for (size_t q = 0; q < n; q++) out[q] = totalcs(*proc, eng[q]);
+    return PT_OK;
+}
+
+int pt_process_desc(const pt_process* p, int slot, pt_proc_desc* out) {
+    if (!p || !out) return host_fail("pt_process_desc: null argument");
+    int rc = check_process(*p);
+    if (rc) return rc;
+    memset(out, 0, sizeof(*out));
+    out->kind = p->kind;
+    out->Z = p->Z;
+    out->slot = slot;
+    out->N = p->N;
+    out->B = p->B;
+    out->U = p->U;
+    out->tcut = p->tcut;
+    out->zf = (double)p->Z;
+    const double Z = (double)p->Z;
+    if (p->kind == PT_PROC_ELASTIC) {
+        out->c[0] = 1.3413 * pow(Z, -1.0 / 3.0) * A0;  // src/relativistic_coulomb.jl:16
+    } else if (p->kind == PT_PROC_BETHE_HEITLER) {
+        // src/bethe_heitler.jl:94-100 and _fc, :153 (the Coulomb correction of alpha, as there)
+        const double az2 = ALPHA * ALPHA, az4 = az2 * az2, az6 = az4 * az2;
+        const double fc = ((((1.0 / (1.0 + az2) + 0.20206) - 0.0369 * az2) + 0.0083 * az4) - 0.0020 * az6) * az2;
+        out->c[0] = pow(Z, 1.0 / 3.0);
+        out->c[1] = 8.0 * log(Z) / 3.0;
+        out->c[2] = out->c[1] + 8.0 * fc;
+    }
+    return PT_OK;
+}
+
+int pt_sb_build(int Z, const double* kfrac, int nx, const double* logT, int ny, const double* values, int ncum,
+                double gamma_min, double gamma_max, double energy_scale, double* log_energy, double* tcs,
+                double* data) {
+    if (!kfrac || !logT || !values || !log_energy || !tcs || !data)
+        return host_fail("pt_sb_build: null argument");
+    if (nx < 2 || ny < 2 || ncum < 2 || Z < 1) return host_fail("pt_sb_build: need nx, ny, ncum >= 2 and Z >= 1");
+    std::vector<double> lk(nx), F(nx);
+    for (int q = 0; q < nx; q++) {
+        if (!(kfrac[q] > 0.0) || (q && !(kfrac[q] > kfrac[q - 1])))
+            return host_fail("pt_sb_build: k/T values must be positive and increasing");
+        lk[q] = log(kfrac[q]);
+    }
+    for (int i = 0; i < ny; i++) {
+        const double* chi = values + (size_t)i * nx;
+        // cumulative of the piecewise-linear interpolant of chi over log(k/T) at the knots;
+        // on each knot interval it is quadratic, so its quantiles have a closed form
+        F[0] = 0.0;
+        for (int q = 1; q < nx; q++) F[q] = F[q - 1] + (chi[q - 1] + chi[q]) * (lk[q] - lk[q - 1]) / 2.0;
+        auto cum_at = [&](double x) {
+            int m = (int)(std::upper_bound(lk.begin(), lk.end(), x) - lk.begin()) - 1;
+            m = std::min(std::max(m, 0), nx - 2);
+            const double d = x - lk[m];
+            const double slope = (chi[m + 1] - chi[m]) / (lk[m + 1] - lk[m]);
+            return F[m] + chi[m] * d + slope * d * d / 2.0;
+        };
+        log_energy[i] = log(exp(logT[i]) * energy_scale);
+        const double T1 = exp(log_energy[i]);
+        const double r = 1.0 + T1 / MC2;
+        const double beta = sqrt(1.0 - 1.0 / (r * r));
+        const double lo = std::max(log(gamma_min / T1), lk[0]);
+        const double hi = std::min(log(gamma_max / T1), lk[nx - 1]);
+        const double c0 = cum_at(lo), c1 = cum_at(hi);
+        double* q = data + (size_t)i * ncum;
+        for (int j = 0; j < ncum; j++) {
+            const double pc = (double)j / (double)(ncum - 1);
+            const double target = c0 + pc * (c1 - c0);
+            int m = (int)(std::upper_bound(F.begin(), F.end(), target) - F.begin()) - 1;
+            m = std::min(std::max(m, 0), nx - 2);
+            const double h = lk[m + 1] - lk[m];
+            const double a = (chi[m + 1] - chi[m]) / (2.0 * h);
+            const double b = chi[m];
+            const double cc = target - F[m];
+            const double d = 2.0 * cc / (b + sqrt(b * b + 4.0 * a * cc));
+            double v = lk[m] + d;
+            v = v < lo ? lo : (v > hi ? hi : v);
+            q[j] = v;
+        }
+        q[0] = lo;
+        q[ncum - 1] = hi;
+        tcs[i] = ((double)Z * (double)Z / (beta * beta)) * 1e-31 * (c1 - c0);
+    }
+    return PT_OK;
+}
+
+int pt_table_build(const pt_process* procs, int nprocs, int species, const double* fdt, int nswarm, double safety,
+                   int nintervals, int order, double max_energy, double* rate, double* ratebound, int32_t* perm,
+                   double* keys) {
+    if (nprocs < 0 || (nprocs > 0 && !procs)) return host_fail("pt_table_build: bad process list");
+    if (species < PT_ELECTRON || species > PT_POSITRON) return host_fail("pt_table_build: bad species");
+    if (nswarm < 1 || !fdt) return host_fail("pt_table_build: need nswarm >= 1 impulses fdt");
+    if (order < 1 || order > MAX_ORDER) return host_fail("pt_table_build: order must be in 1..16");
+    if (nintervals < 1 || nintervals > MAX_K) return host_fail("pt_table_build: nintervals must be in 1..63");
+    if (!(max_energy > 0.0)) return host_fail("pt_table_build: max_energy must be > 0");
+    for (int j = 0; j < nprocs; j++) {
+        int rc = check_process(procs[j]);
+        if (rc) return rc;
+    }
+    const Intervals iv{nintervals, max_energy};
+    const int K1 = nintervals + 1, L = nprocs, n = order;
+    // per-process fits in the given order: a[j][(k+1)][n]
+    std::vector<double> a((size_t)L * K1 * n);
+    for (int j = 0; j < L; j++) {
+        const pt_process& p = procs[j];
+        auto f = [&](double e) { return p.coef * speed_of(species, e) * totalcs(p, e); };
+        if (!chebfit(iv, n, f, a.data() + (size_t)j * K1 * n)) return host_fail("pt_table_build: singular fit");
+    }
+    // compratebound (src/collision_table.jl:152-167), summed in the given order
+    if (ratebound) {
+        for (int s = 0; s < nswarm; s++) {
+            const double Fdt = fdt[s];
+            auto f = [&](double e) {
+                double xi;
+                const int i = iv.locate(e, &xi);
+                double lo, hi;
+                iv.bounds(i, &lo, &hi);
+                double fv = 0.0, df = 0.0;
+                for (int j = 0; j < L; j++) {
+                    const double* c = a.data() + ((size_t)j * K1 + i) * n;
+                    const double v = cheb_value(c, n, xi), d = cheb_slope(c, n, xi, lo, hi);
+                    fv = j ? fv + v : v;
+                    df = j ? df + d : d;
+                }
+                const double g = species == PT_PHOTON ? 0.0 : 1.0 + e / MC2;
+                return fv + g * g * g * speed_of(species, e) * Fdt * fabs(df) * safety;
+            };
+            if (!chebfit(iv, n, f, ratebound + (size_t)s * K1 * n))
+                return host_fail("pt_table_build: singular fit");
+        }
+    }
+    // process order (src/collision_table.jl:137-146): the largest coef * totalcs * v on a
+    // log grid first, NaN keys before every number, ties in the given order
+    std::vector<double> key(L);
+    {
+        const double L1 = log(1e-2 * EV), L2 = log(0.9999 * max_energy);
+        const double e0 = exp(L1);
+        std::vector<double> eng(SORT_POINTS), v1(SORT_POINTS);
+        for (int q = 0; q < SORT_POINTS; q++) {
+            const double t = (double)q / (double)(SORT_POINTS - 1);
+            eng[q] = exp((1.0 - t) * L1 + t * L2) - e0;
+            v1[q] = speed_of(species, eng[q]);
+        }
+        for (int j = 0; j < L; j++) {
+            double mx = -INFINITY;
+            bool nan = false;
+            for (int q = 0; q < SORT_POINTS && !nan; q++) {
+                const double v = totalcs(procs[j], eng[q]) * v1[q];
+                if (isnan(v))
+                    nan = true;
+                else if (v > mx)
+                    mx = v;
+            }
+            key[j] = procs[j].coef * (nan ? NAN : mx);
+        }
+    }
+    std::vector<int> order_idx(L);
+    for (int j = 0; j < L; j++) order_idx[j] = j;
+    std::stable_sort(order_idx.begin(), order_idx.end(), [&](int x, int y) {
+        const bool nx = isnan(key[x]), ny = isnan(key[y]);
+        if (nx || ny) return nx && !ny;
+        return key[x] > key[y];
+    });
+    for (int pos = 0; pos < L; pos++) {
+        const int j = order_idx[pos];
+        if (perm) perm[pos] = j;
+        if (keys) keys[pos] = key[j];
+        if (rate)
+            for (int i = 0; i < K1; i++)
+                for (int o = 0; o < n; o++)
+                    rate[((size_t)i * L + pos) * n + o] = a[((size_t)j * K1 + i) * n + o];
+    }
+    return PT_OK;
+}
+
+}  // extern "C"

@@ -127,6 +127,78 @@ function upload_table!(m::DeviceMultiPopulation, species, tbl, energy_cut)
                                                   (Ptr{Cvoid}, Cint, Ref{PtTableDesc}), m.ctx, species, d))
 end
 
+struct PtProcess
+    kind::Int32; Z::Int32; N::Int32; ntab::Int32
+    coef::Float64; B::Float64; U::Float64; tcut::Float64
+    tab_x::Ptr{Float64}; tab_y::Ptr{Float64}
+end
+
+# the raw process the library's table builder takes (pointers into arrays `p` keeps)
+function ptprocess(coef, p)
+    kind = PROC_KIND[typeof(p).name.wrapper]
+    Z = hasproperty(p, :Z) ? Int32(p.Z) : Int32(0)
+    N = hasproperty(p, :N) ? Int32(p.N) : Int32(0)
+    B = hasproperty(p, :B) ? p.B : 0.0
+    U = hasproperty(p, :U) ? p.U : 0.0
+    tcut = hasproperty(p, :tcut) ? p.tcut : 0.0
+    ntab, x, y = Int32(0), Ptr{Float64}(C_NULL), Ptr{Float64}(C_NULL)
+    if kind == 2          # SeltzerBerger: its tabulated totals
+        ntab, x, y = Int32(length(p.log_energy)), pointer(p.log_energy), pointer(p.totalcs)
+    elseif kind == 6      # PhotoElectric: Sandia intervals, coeffs is (4, n)
+        ntab, x, y = Int32(length(p.left_energy)), pointer(p.left_energy), pointer(p.coeffs)
+    end
+    return PtProcess(kind, Z, N, ntab, Float64(coef), B, U, tcut, x, y)
+end
+
+"""
+    build_table!(m, species, processes, Fdts; safety, nintervals, order, max_energy, energy_cut)
+
+collision_table_from_processes built by the library (pt_table_build) from the reference's
+`(coef, process)` pairs, with one rate bound per impulse in `Fdts` (one per swarm), and
+uploaded with the processes' Seltzer-Berger and photoelectric data.
+"""
+function build_table!(m::DeviceMultiPopulation, species, processes, Fdts; safety=1.1, nintervals=32, order=3,
+                      max_energy=1e3 * Particulator.co.eV * 2^18, energy_cut=0.0)
+    L = length(processes)
+    raw = [ptprocess(c, p) for (c, p) in processes]
+    fdt = Float64.(collect(Fdts))
+    rate = zeros(Float64, order * L * (nintervals + 1))
+    rb = zeros(Float64, order * (nintervals + 1) * length(fdt))
+    perm = zeros(Int32, L)
+    GC.@preserve processes raw fdt rate rb perm check(C_NULL, ccall((:pt_table_build, libpt), Cint,
+        (Ptr{PtProcess}, Cint, Cint, Ptr{Float64}, Cint, Cdouble, Cint, Cint, Cdouble, Ptr{Float64},
+         Ptr{Float64}, Ptr{Int32}, Ptr{Float64}),
+        raw, L, species, fdt, length(fdt), safety, nintervals, order, max_energy, rate, rb, perm, C_NULL))
+    procs = PtProcDesc[]
+    sbslot = 0
+    peslot = 0
+    for pos in 1:L
+        p = processes[perm[pos] + 1][2]
+        slot = 0
+        if raw[perm[pos] + 1].kind == 2
+            slot = sbslot; sbslot += 1
+            data = Matrix(p.data)
+            check(m.ctx, ccall((:pt_upload_sb, libpt), Cint,
+                               (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Ptr{Float64}, Ptr{Float64}),
+                               m.ctx, slot, p.Z, length(p.log_energy), size(data, 1), p.log_energy, data))
+        elseif raw[perm[pos] + 1].kind == 6
+            slot = peslot; peslot += 1
+            check(m.ctx, ccall((:pt_upload_pe, libpt), Cint, (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}),
+                               m.ctx, slot, p.Z, length(p.binding), p.binding))
+        end
+        d = Ref{PtProcDesc}()
+        GC.@preserve processes check(C_NULL, ccall((:pt_process_desc, libpt), Cint,
+            (Ref{PtProcess}, Cint, Ref{PtProcDesc}), raw[perm[pos] + 1], slot, d))
+        push!(procs, d[])
+    end
+    push!(m.keep, (procs, rb, rate))
+    td = Ref(PtTableDesc(L, order, nintervals, length(fdt), max_energy, energy_cut,
+                         pointer(rate), pointer(rb), pointer(procs)))
+    GC.@preserve procs rb rate check(m.ctx, ccall((:pt_upload_table, libpt), Cint,
+                                                  (Ptr{Cvoid}, Cint, Ref{PtTableDesc}), m.ctx, species, td))
+    return perm .+ 1
+end
+
 fieldfn(f::HomogeneousField) = PtFieldFn(1, 0, Tuple(f.v), (0.0, 0.0, 0.0), 0, 0, 0, 0, 0, 0)
 fieldfn(f::DoubleLayerField) = PtFieldFn(2, 0, Tuple(f.v), (0.0, 0.0, 0.0), f.z1, f.z2, 0, 0, 0, 0)
 fieldfn(f::StepField) = PtFieldFn(3, 0, Tuple(f.v1), Tuple(f.v2), f.z, 0, 0, 0, 0, 0)

@@ -37,6 +37,13 @@ class ProcDesc(C.Structure):
                 ("c", C.c_double * 4)]
 
 
+class Process(C.Structure):
+    """pt_process: a raw process for the host table builder"""
+    _fields_ = [("kind", C.c_int32), ("Z", C.c_int32), ("N", C.c_int32), ("ntab", C.c_int32),
+                ("coef", C.c_double), ("B", C.c_double), ("U", C.c_double), ("tcut", C.c_double),
+                ("tab_x", C.POINTER(C.c_double)), ("tab_y", C.POINTER(C.c_double))]
+
+
 class TableDesc(C.Structure):
     _fields_ = [("nprocs", C.c_int32), ("order", C.c_int32), ("nintervals", C.c_int32),
                 ("nswarm", C.c_int32), ("xmax", C.c_double), ("energy_cut", C.c_double),
@@ -117,6 +124,12 @@ SIGNATURES = {
     "pt_destroy": (None, [VP]),
     "pt_last_error": (C.c_char_p, [VP]),
     "pt_upload_table": (C.c_int, [VP, C.c_int, C.POINTER(TableDesc)]),
+    "pt_totalcs": (C.c_int, [C.POINTER(Process), DP, C.c_size_t, DP]),
+    "pt_process_desc": (C.c_int, [C.POINTER(Process), C.c_int, C.POINTER(ProcDesc)]),
+    "pt_sb_build": (C.c_int, [C.c_int, DP, C.c_int, DP, C.c_int, DP, C.c_int, C.c_double, C.c_double,
+                              C.c_double, DP, DP, DP]),
+    "pt_table_build": (C.c_int, [C.POINTER(Process), C.c_int, C.c_int, DP, C.c_int, C.c_double, C.c_int,
+                                 C.c_int, C.c_double, DP, DP, C.POINTER(C.c_int32), DP]),
     "pt_upload_sb": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, C.c_int, DP, DP]),
     "pt_upload_pe": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, DP]),
     "pt_set_field": (C.c_int, [VP, C.c_int, C.POINTER(FieldDesc)]),
@@ -194,6 +207,6 @@ def lib():
 
 def check(ctx, rc):
     if rc != 0:
-        msg = lib().pt_last_error(ctx).decode() if ctx else ""
+        msg = lib().pt_last_error(ctx).decode()
         raise PTError(rc, msg)
     return rc

@@ -19,7 +19,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
 
 
 def sources():
-    return [os.path.join(CSRC, "particulator.hip")]
+    return [os.path.join(CSRC, "particulator.hip"), os.path.join(CSRC, "pt_tables.cpp")]
 
 
 def build(verbose=False):

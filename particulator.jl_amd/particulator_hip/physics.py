@@ -5,6 +5,7 @@ Each class mirrors the reference type of the same name; `totalcs(E)` is vectoris
 numpy array of kinetic energies (J) and evaluates the reference's formula in its order.
 The secondary samplers themselves run on the GPU (csrc/pt_device.h).
 """
+import ctypes as C
 import math
 import os
 
@@ -13,7 +14,11 @@ import numpy as np
 from . import constants as co
 from ._lib import (PROC_ANNIHILATION, PROC_BETHE_HEITLER, PROC_BHABHA, PROC_COMPTON,
                    PROC_ELASTIC, PROC_KN_COMPTON, PROC_MOLLER, PROC_PHOTOELECTRIC, PROC_RBEB,
-                   PROC_SB, ProcDesc)
+                   PROC_SB, Process, ProcDesc, check, lib)
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
 
 DATA_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "data")
 
@@ -37,6 +42,22 @@ class CollisionProcess:
         d.Z = int(self.Z)
         d.zf = float(self.Z)
         return d
+
+    def process(self, coef=1.0):
+        """the pt_process of this process (a raw process for the library's table builder);
+        array pointers refer to arrays this object keeps"""
+        p = Process()
+        p.kind = self.kind
+        p.Z = int(self.Z)
+        p.coef = float(coef)
+        return p
+
+    def totalcs_native(self, E):
+        """totalcs evaluated by the library's host code (pt_totalcs)"""
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        out = np.empty_like(E)
+        check(None, lib().pt_totalcs(C.byref(self.process()), _dp(E), E.size, _dp(out)))
+        return out
 
 
 class RelativisticCoulomb(CollisionProcess):
@@ -100,6 +121,11 @@ class RBEB(CollisionProcess):
         d.B, d.U, d.N = self.B, self.U, self.N
         return d
 
+    def process(self, coef=1.0):
+        p = super().process(coef)
+        p.B, p.U, p.N = self.B, self.U, int(self.N)
+        return p
+
 
 # molecular orbitals (B, U in eV; N), src/rbeb.jl:25-48
 N2_ORBITALS = [RBEB(0.4095e3 * co.eV, 0.6033e3 * co.eV, 4), RBEB(41.72 * co.eV, 71.13 * co.eV, 2),
@@ -137,6 +163,11 @@ class Moller(CollisionProcess):
         d.tcut = self.tcut
         return d
 
+    def process(self, coef=1.0):
+        p = super().process(coef)
+        p.tcut = self.tcut
+        return p
+
 
 def _bhabha_bs(y):
     q = 1.0 - 2.0 * y
@@ -169,6 +200,11 @@ class Bhaba(CollisionProcess):
         d = super().desc()
         d.tcut = self.tcut
         return d
+
+    def process(self, coef=1.0):
+        p = super().process(coef)
+        p.tcut = self.tcut
+        return p
 
 
 Bhabha = Bhaba
@@ -365,6 +401,13 @@ class PhotoElectric(CollisionProcess):
                 acc = acc + ep[i] * self.coeffs[jj, i]
         return np.where(j >= 0, acc, 0.0)
 
+    def process(self, coef=1.0):
+        p = super().process(coef)
+        self._tab = (np.ascontiguousarray(self.left_energy), np.ascontiguousarray(self.coeffs))
+        p.ntab = self.left_energy.size
+        p.tab_x, p.tab_y = _dp(self._tab[0]), _dp(self._tab[1])
+        return p
+
     def desc(self, slots=None):
         d = super().desc()
         d.slot = slots.pe_slot(self) if slots is not None else 0
@@ -422,15 +465,24 @@ class SeltzerBerger(CollisionProcess):
     kind = PROC_SB
 
     def __init__(self, Z, ncum=1000, gamma_min=1e2 * co.eV, gamma_max=5e7 * co.eV,
-                 energy_scale=1e6 * co.eV, path=None):
+                 energy_scale=1e6 * co.eV, path=None, native=False):
+        """native: build the inverse CDFs with the library's host code (pt_sb_build)
+        instead of numpy"""
         self.Z = Z
         path = path or os.path.join(DATA_DIR, "brem_SB", "br%d" % Z)
         x, y, vals = read_g4_2d(path)
-        self.log_energy = np.log(np.exp(y) * energy_scale)
         self.pcum = np.arange(ncum, dtype=np.float64) / (ncum - 1)
         ne = y.size
         self.data = np.zeros((ne, ncum))
         self.tcs = np.zeros(ne)
+        if native:
+            self.log_energy = np.zeros(ne)
+            x, y, vals = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, vals))
+            check(None, lib().pt_sb_build(int(Z), _dp(x), x.size, _dp(y), ne, _dp(vals), ncum, gamma_min,
+                                          gamma_max, energy_scale, _dp(self.log_energy), _dp(self.tcs),
+                                          _dp(self.data)))
+            return
+        self.log_energy = np.log(np.exp(y) * energy_scale)
         logk = np.log(x)
         mc2 = co.electron_mc2
         for i in range(ne):
@@ -452,6 +504,12 @@ class SeltzerBerger(CollisionProcess):
         w = (le[ii + 1] - lk) / (le[ii + 1] - le[ii])
         v = w * self.tcs[ii] + (1.0 - w) * self.tcs[ii + 1]
         return np.where(i >= 0, v, 0.0)
+
+    def process(self, coef=1.0):
+        p = super().process(coef)
+        p.ntab = self.log_energy.size
+        p.tab_x, p.tab_y = _dp(self.log_energy), _dp(self.tcs)
+        return p
 
     def desc(self, slots=None):
         d = super().desc()

@@ -10,7 +10,7 @@ import math
 import numpy as np
 
 from . import constants as co
-from ._lib import PT_PHOTON, ProcDesc, TableDesc
+from ._lib import PT_PHOTON, Process, ProcDesc, TableDesc, check, lib
 
 DEFAULT_MAX_ENERGY = 1e3 * co.eV * 2 ** 18  # src/collision_table.jl:122
 
@@ -205,4 +205,28 @@ def collision_table_from_processes(processes, species, Fdt, safety=1.1, ninterva
     t = ChebyshevCollisionTable(procs, b, a[:, perm, :], rb, species, energy_cut,
                                 [keys[p] for p in perm])
     t._fit = a  # per-process fits in the original order, for ratebound_for
+    return t
+
+
+def native_table(processes, species, Fdt, safety=1.1, nintervals=32, order=3, max_energy=DEFAULT_MAX_ENERGY,
+                 energy_cut=0.0):
+    """collision_table_from_processes built by the library's host code (pt_table_build)
+    from (coef, process) pairs.  Fdt may be a sequence: one rate bound per swarm."""
+    fdts = np.ascontiguousarray(np.atleast_1d(np.asarray(Fdt, dtype=np.float64)))
+    L = len(processes)
+    arr = (Process * max(L, 1))(*[p.process(coef) for coef, p in processes])
+    rate = np.zeros((nintervals + 1, L, order))
+    rb = np.zeros((fdts.size, nintervals + 1, order))
+    perm = np.zeros(max(L, 1), dtype=np.int32)
+    keys = np.zeros(max(L, 1))
+    dp = C.POINTER(C.c_double)
+    check(None, lib().pt_table_build(arr, L, species, fdts.ctypes.data_as(dp), fdts.size, safety, nintervals,
+                                     order, max_energy, rate.ctypes.data_as(dp), rb.ctypes.data_as(dp),
+                                     perm.ctypes.data_as(C.POINTER(C.c_int32)), keys.ctypes.data_as(dp)))
+    perm = perm[:L]
+    t = ChebyshevCollisionTable([processes[i][1] for i in perm], BinaryIntervals(nintervals, max_energy), rate,
+                                rb, species, energy_cut, list(keys[:L]))
+    fit = np.zeros_like(rate)
+    fit[:, perm, :] = rate
+    t._fit = fit  # per-process fits in the given order, for ratebound_for
     return t

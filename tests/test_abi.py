@@ -41,7 +41,9 @@ def test_ctypes_layouts_match_the_header():
     from particulator_hip import _lib as L
     structs = {"pt_proc_desc": L.ProcDesc, "pt_table_desc": L.TableDesc, "pt_field_fn": L.FieldFn,
                "pt_field_desc": L.FieldDesc, "pt_config": L.Config, "pt_pop_stats": L.PopStats,
-               "pt_work": L.Work}
+               "pt_work": L.Work, "pt_process": L.Process, "pt_launch": L.Launch,
+               "pt_wall_record": L.WallRecord, "pt_tune_trial": L.TuneTrial, "pt_learn_config": L.LearnConfig,
+               "pt_learn_report": L.LearnReport}
     src = ["#include <stdio.h>", "#include <stddef.h>", '#include "%s"' % HEADER, "int main(void){"]
     for cname, cls in structs.items():
         src.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))

@@ -1,0 +1,152 @@
+"""The library's host table builder (pt_totalcs, pt_sb_build, pt_table_build,
+pt_process_desc; no device needed) against the numpy builder of the host package and the
+oracle's C builder: cross sections, Seltzer-Berger inverse CDFs, Chebyshev rate tables,
+rate bounds per swarm, process order and the process descriptors."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from particulator_hip import PTError, air, co, physics, tables
+from particulator_hip._lib import PT_ELECTRON, PT_PHOTON, PT_POSITRON, ProcDesc, Process, lib
+
+CUT = 1e4 * co.eV
+
+
+def species_processes():
+    comp = air.composition()
+    F = co.e * 1e6 * 1e-12
+    return [(PT_ELECTRON, air.electron_processes(comp, CUT), F), (PT_PHOTON, air.photon_processes(comp), 0.0),
+            (PT_POSITRON, air.positron_processes(comp, CUT), F)]
+
+
+def test_cross_sections_match_numpy():
+    """every process of the three air lists, 1e-3 eV-300 MeV and E = 0: same NaNs, same
+    zeros, values within a few ulps (libm against numpy's own log/exp/pow)"""
+    E = np.concatenate([[0.0], np.geomspace(1e-3 * co.eV, 3e8 * co.eV, 20001)])
+    seen = set()
+    for _, procs, _ in species_processes():
+        for _, p in procs:
+            with np.errstate(all="ignore"):
+                x = p.totalcs(E)
+            y = p.totalcs_native(E)
+            assert np.array_equal(np.isnan(x), np.isnan(y)), type(p).__name__
+            assert np.array_equal(x == 0, y == 0), type(p).__name__
+            m = ~np.isnan(x) & (x != 0)
+            assert np.all(np.abs(x[m] - y[m]) <= 2e-13 * np.abs(x[m])), type(p).__name__
+            seen.add(p.kind)
+    assert len(seen) == 9
+
+
+def test_cross_section_closed_forms():
+    """Klein-Nishina 0.2112 b per electron at 1 MeV; pair production zero below 2 mc^2;
+    Moller zero below 2 tcut (the secondary must carry at least tcut)"""
+    kn = physics.KleinNishinaCompton(1).totalcs_native(np.array([1e6 * co.eV]))[0]
+    assert abs(kn / 1e-28 - 0.2112) < 5e-4
+    bh = physics.BetheHeitler(7).totalcs_native(np.array([1.02e6 * co.eV, 1.03e6 * co.eV]))
+    assert bh[0] == 0.0 and bh[1] > 0.0
+    m = physics.Moller(18, CUT).totalcs_native(np.array([0.99 * CUT, 2.5 * CUT]))
+    assert m[0] == 0.0 and m[1] > 0.0
+
+
+@pytest.mark.parametrize("Z", [7, 8, 18])
+def test_sb_build_matches_numpy_and_oracle(O, Z):
+    a = physics.SeltzerBerger(Z)
+    b = physics.SeltzerBerger(Z, native=True)
+    assert np.array_equal(a.log_energy, b.log_energy)
+    assert np.abs(a.data - b.data).max() <= 1e-13
+    assert np.allclose(a.tcs, b.tcs, rtol=1e-13, atol=0)
+    o = O.sb(Z)  # the oracle's Newton build
+    assert np.abs(b.data - o.data).max() < 1e-9
+    # quantiles increase and span the requested photon range
+    assert np.all(np.diff(b.data, axis=1) >= 0)
+
+
+@pytest.mark.parametrize("sp", [PT_ELECTRON, PT_PHOTON, PT_POSITRON])
+def test_table_matches_numpy_and_oracle(O, sp):
+    _, procs, F = species_processes()[sp]
+    a = tables.collision_table_from_processes(procs, sp, F, energy_cut=CUT)
+    b = tables.native_table(procs, sp, F, energy_cut=CUT)
+    assert all(p is q for p, q in zip(a.proc, b.proc))
+    ka, kb = np.array(a.sort_keys), np.array(b.sort_keys)
+    assert np.array_equal(np.isnan(ka), np.isnan(kb))
+    assert np.allclose(ka[~np.isnan(ka)], kb[~np.isnan(kb)], rtol=1e-14, atol=0)
+    big = np.abs(a.rate) > 1e-6 * np.abs(a.rate).max()
+    assert (np.abs(a.rate - b.rate)[big] / np.abs(a.rate)[big]).max() < 1e-11
+    assert (np.abs(a.ratebound - b.ratebound) / np.abs(a.ratebound).clip(1e-300)).max() < 1e-11
+    ot = O.OracleTable(O.air_processes(sp, O.air_composition(), CUT), sp, F)
+    assert [p.kind for p in b.proc] == [ot.procs[j].kind for j in range(len(ot.procs))]
+    big = np.abs(ot.rate) > 1e-6 * np.abs(ot.rate).max()
+    assert (np.abs(b.rate - ot.rate)[big] / np.abs(ot.rate)[big]).max() < 1e-11
+
+
+def test_table_interpolates_the_rates_at_the_nodes():
+    """chebfit's defining property (src/cheby.jl:211-227): at the Chebyshev nodes of each
+    interval the fitted rate equals coef * v * totalcs"""
+    sp, procs, F = species_processes()[PT_ELECTRON]
+    t = tables.native_table(procs, sp, F, energy_cut=CUT)
+    b = t.b
+    worst = 0.0
+    for i in range(1, b.k + 1):
+        lo, hi = b.interval(i)
+        x = tables.chebnodes(3, lo, hi)
+        for j, p in enumerate(t.proc):
+            coef = next(c for c, q in procs if q is p)
+            exact = coef * tables.speed(sp, x) * p.totalcs_native(x)
+            fit = t.rate_of(j, x)
+            scale = np.abs(t.rate[i, j]).max()
+            if scale > 0:
+                worst = max(worst, np.abs(fit - exact).max() / scale)
+    assert worst < 1e-12
+
+
+def test_rate_bounds_per_swarm():
+    """one build with three impulses gives, bit for bit, the bounds of three one-field
+    builds, and they agree with the numpy builder's ratebound_for"""
+    sp, procs, _ = species_processes()[PT_ELECTRON]
+    Fs = [co.e * E * 1e-12 for E in (4e5, 1e6, 4e6)]
+    t3 = tables.native_table(procs, sp, Fs, energy_cut=CUT)
+    assert t3.ratebound.shape[0] == 3
+    a = tables.collision_table_from_processes(procs, sp, Fs[0], energy_cut=CUT)
+    for s, F in enumerate(Fs):
+        t1 = tables.native_table(procs, sp, F, energy_cut=CUT)
+        assert np.array_equal(t3.ratebound[s], t1.ratebound[0])
+        ref = a.ratebound_for(F)
+        assert (np.abs(ref - t3.ratebound[s]) / np.abs(ref).clip(1e-300)).max() < 1e-11
+        # the native table's own ratebound_for (fits kept in the given order) is the same build
+        assert np.allclose(t1.ratebound_for(F), t1.ratebound[0], rtol=1e-12, atol=0)
+
+
+def test_process_descriptors_match_host():
+    for _, procs, _ in species_processes():
+        for _, p in procs:
+            d = ProcDesc()
+            rc = lib().pt_process_desc(C.byref(p.process()), 0, C.byref(d))
+            assert rc == 0
+            e = p.desc()
+            for f in ("kind", "Z", "N", "B", "U", "tcut", "zf"):
+                assert getattr(d, f) == getattr(e, f), (type(p).__name__, f)
+            assert list(d.c) == list(e.c), type(p).__name__
+
+
+def test_errors_are_reported_without_a_context():
+    bad = Process()
+    bad.kind = 99
+    bad.Z = 7
+    E = np.array([1e-13])
+    out = np.zeros(1)
+    dp = C.POINTER(C.c_double)
+    rc = lib().pt_totalcs(C.byref(bad), E.ctypes.data_as(dp), 1, out.ctypes.data_as(dp))
+    assert rc == -1
+    assert "unknown process kind" in lib().pt_last_error(None).decode()
+    sb = Process()
+    sb.kind = physics.PROC_SB
+    sb.Z = 7
+    with pytest.raises(PTError) as e:
+        tables.native_table([(1.0, type("P", (), {"process": lambda self, c: sb})())], PT_ELECTRON, 0.0)
+    assert "pt_sb_build" in str(e.value)
+    _, procs, F = species_processes()[PT_ELECTRON]
+    with pytest.raises(PTError) as e:
+        tables.native_table(procs, PT_ELECTRON, F, order=0)
+    assert "order" in str(e.value)
