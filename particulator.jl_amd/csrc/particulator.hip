@@ -168,6 +168,8 @@ struct Shared {
     SpLite sp[PT_NSPECIES];
     DevProc procs[PT_NSPECIES][MAX_PROCS];
     BlockCounters bc;
+    double cut[PT_NSPECIES];  // new particles at or below the cut are not stored
+    int stored[PT_NSPECIES];  // species whose new particles are stored (tracked, with a table)
 };
 
 __device__ __forceinline__ void stage(const Params& P, double* lds, Shared& sh) {
@@ -196,6 +198,8 @@ __device__ __forceinline__ void stage(const Params& P, double* lds, Shared& sh) 
             L.xmax = tb.xmax;
             L.rate = lds + P.lds_rate_off[s];
             L.rb = P.rb_in_lds ? lds + P.lds_rb_off[s] : tb.ratebound;
+            sh.cut[s] = tb.cut;
+            sh.stored[s] = S.tracked && tb.valid;
         }
         if (!tb.valid) continue;
         for (int q = threadIdx.x; q < tb.nprocs; q += blockDim.x) sh.procs[s][q] = tb.procs[q];
@@ -252,9 +256,13 @@ __device__ __forceinline__ void deposit(const Params& P, BlockCounters& bc, doub
 constexpr ull NO_SLOT = ~0ull;
 constexpr int NCHILD = 2;
 
+// add_particle! (src/population.jl:99-122) of the k-th new particle of the event of
+// particle puid that ended at stream position dend; its uid is computed only when it is
+// stored
 __device__ __forceinline__ ull append(const Params& P, Shared& sh, int sp2, const V3& x,
-                                      const V3& p, double w, double t, double s, uint64_t uid,
-                                      int swarm, bool main_pass, double tfinal) {
+                                      const V3& p, double w, double t, double s, uint64_t puid,
+                                      uint32_t step, uint32_t dend, int k, int swarm, bool main_pass,
+                                      double tfinal) {
     const SpLite& L = sh.sp[sp2];
     double E = kinenergy(sp2, p);
     if (E <= L.cut) {
@@ -296,7 +304,7 @@ __device__ __forceinline__ ull append(const Params& P, Shared& sh, int sp2, cons
     L.f[PT_T][slot] = t;
     L.f[PT_S][slot] = s;
     L.f[PT_R][slot] = r;
-    L.uid[slot] = uid;
+    L.uid[slot] = child_uid(P.seed, puid, step, dend, k);
     L.swarm[slot] = swarm;
     L.active[slot] = 1;
     L.trem[slot] = tfinal - t;
@@ -607,7 +615,7 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const u
                                     if (e.dep != 0.0) deposit(P, sh.bc, e.dep * w, x.z);
                                     V3 m0, m1;
                                     double s0, s1;
-                                    finish_event(p, pn, e, g, m0, m1, s0, s1);
+                                    finish_event(p, pn, e, g, m0, m1, s0, s1, sh.cut, sh.stored);
                                     uint32_t dend = g.draw;
                                     if (e.keep) {
                                         p = m0;
@@ -615,9 +623,8 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const u
                                         r = setr(L, kinenergy(sp, p), swarm, oob);
                                     } else {
                                         active = false;
-                                        ull sl = append(P, sh, e.sp0, x, m0, w, t, s0,
-                                                        child_uid(P.seed, uid, step, dend, 0), swarm, main_pass,
-                                                        tfinal);
+                                        ull sl = append(P, sh, e.sp0, x, m0, w, t, s0, uid, step, dend, 0,
+                                                        swarm, main_pass, tfinal);
                                         if (sl != NO_SLOT && nch < NCHILD) {
                                             if (nch == 0) {
                                                 chsp[0] = e.sp0;
@@ -630,9 +637,8 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const u
                                         }
                                     }
                                     if (e.ne == 2) {
-                                        ull sl = append(P, sh, e.sp1, x, m1, w, t, s1,
-                                                        child_uid(P.seed, uid, step, dend, e.keep ? 0 : 1), swarm,
-                                                        main_pass, tfinal);
+                                        ull sl = append(P, sh, e.sp1, x, m1, w, t, s1, uid, step, dend,
+                                                        e.keep ? 0 : 1, swarm, main_pass, tfinal);
                                         if (sl != NO_SLOT && nch < NCHILD) {
                                             if (nch == 0) {
                                                 chsp[0] = e.sp1;
@@ -918,7 +924,7 @@ __global__ void __launch_bounds__(BS) k_sample(const Params* __restrict__ Pp) {
             if (e.dep != 0.0) deposit(P, sh.bc, e.dep * w, x.z);
             V3 m0, m1;
             double s0, s1;
-            finish_event(p, pn, e, g, m0, m1, s0, s1);
+            finish_event(p, pn, e, g, m0, m1, s0, s1, sh.cut, sh.stored);
             const uint32_t dend = g.draw;
             if (e.keep) {
                 bool oob = false;
@@ -936,8 +942,7 @@ __global__ void __launch_bounds__(BS) k_sample(const Params* __restrict__ Pp) {
             } else {
                 L.active[idx] = 0;
                 atomicAdd(&sh.bc.work[child ? W_TAIL : W_PSTEPS], 1ull);
-                ull sl = append(P, sh, e.sp0, x, m0, w, t, s0, child_uid(P.seed, uid, step, dend, 0), swarm,
-                                false, tfinal);
+                ull sl = append(P, sh, e.sp0, x, m0, w, t, s0, uid, step, dend, 0, swarm, false, tfinal);
                 if (sl != NO_SLOT) {
                     c1 = P.q.cnt + QWK(e.sp0, nxt);
                     a1 = P.q.work[e.sp0][nxt];
@@ -945,8 +950,8 @@ __global__ void __launch_bounds__(BS) k_sample(const Params* __restrict__ Pp) {
                 }
             }
             if (e.ne == 2) {
-                ull sl = append(P, sh, e.sp1, x, m1, w, t, s1, child_uid(P.seed, uid, step, dend, e.keep ? 0 : 1),
-                                swarm, false, tfinal);
+                ull sl = append(P, sh, e.sp1, x, m1, w, t, s1, uid, step, dend, e.keep ? 0 : 1, swarm, false,
+                                tfinal);
                 if (sl != NO_SLOT) {
                     c2 = P.q.cnt + QWK(e.sp1, nxt);
                     a2 = P.q.work[e.sp1][nxt];
@@ -1782,7 +1787,7 @@ __global__ void __launch_bounds__(256) k_dbg_collide(const Params* __restrict__ 
     sample_event(P.tab[sp].procs[proc], sp, p, pn, E, P.sb, P.pe, g, e);
     V3 m0, m1;
     double s0, s1;
-    finish_event(p, pn, e, g, m0, m1, s0, s1);
+    finish_event(p, pn, e, g, m0, m1, s0, s1, nullptr, nullptr);
     int kind = e.keep ? (e.ne == 1 ? 1 : 2) : (e.ne == 1 ? 3 : 4);
     okind[i] = g.ran_out ? -2 : (e.err ? -1 - 16 * e.err : kind);
     odraws[i] = (int32_t)g.draw;

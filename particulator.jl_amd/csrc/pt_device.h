@@ -81,8 +81,11 @@ __device__ __forceinline__ void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c
             k0 += 0x9E3779B9u;
             k1 += 0xBB67AE85u;
         }
-        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        // one 32x32->64 product per word (a single v_mad_u64_u32) instead of separate
+        // high and low multiplies
+        const uint64_t m0 = (uint64_t)0xD2511F53u * c0, m1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(m0 >> 32), lo0 = (uint32_t)m0;
+        uint32_t hi1 = (uint32_t)(m1 >> 32), lo1 = (uint32_t)m1;
         uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
         c0 = n0;
         c1 = lo1;
@@ -113,6 +116,9 @@ struct PhiloxStream {
         draw = 0;
         cblk = 0;
     }
+    // advance past one draw whose value is not needed (the block is computed by the next
+    // draw that needs it)
+    __device__ __forceinline__ void skip() { draw++; }
     __device__ __forceinline__ double next() {
         uint32_t d = draw++;
         uint32_t blk = d >> 1;
@@ -134,6 +140,7 @@ struct InjectedStream {
     uint32_t draw;
     bool ran_out;
     __device__ bool exhausted() const { return ran_out; }
+    __device__ __forceinline__ void skip() { (void)next(); }
     __device__ __forceinline__ double next() {
         draw++;
         if (pos >= n) {
@@ -849,10 +856,14 @@ __device__ __forceinline__ void learned_event(const DevProc& pr, int sp, double 
 }
 
 // turn (src/util.jl:40-57) of p by turn A and, if requested, turn B; momenta of the
-// outgoing entries and the constructors' s draws (src/electron.jl:32, src/photon.jl:30)
+// outgoing entries and the constructors' s draws (src/electron.jl:32, src/photon.jl:30).
+// cut[sp] / tracked[sp] (NULL: every s is computed) tell which new particles add_particle!
+// drops (at or below the cut, or not tracked): their s is never read, so its draw only
+// advances the stream.
 template <class G>
 __device__ __forceinline__ void finish_event(const V3& p, double pnorm, const Ev& e, G& g, V3& m0,
-                                             V3& m1, double& s0, double& s1) {
+                                             V3& m1, double& s0, double& s1, const double* cut,
+                                             const int* tracked) {
     double u0 = p.x / pnorm, u1 = p.y / pnorm, u2 = p.z / pnorm;
     double sp, cp;
     sincos(e.phi, &sp, &cp);
@@ -897,10 +908,24 @@ __device__ __forceinline__ void finish_event(const V3& p, double pnorm, const Ev
         pb = V3{0.0, 0.0, 0.0};
     }
     V3 pma{p.x - pa.x, p.y - pa.y, p.z - pa.z};
-    s0 = nextcoll(g);
-    s1 = e.ne == 2 ? nextcoll(g) : 0.0;
     m0 = e.src0 == SRC_A ? pa : (e.src0 == SRC_B ? pb : pma);
     m1 = e.src1 == SRC_A ? pa : (e.src1 == SRC_B ? pb : pma);
+    auto stored = [&](int spk, const V3& m) {
+        return !cut || (tracked[spk] && kinenergy(spk, m) > cut[spk]);
+    };
+    if (e.keep || stored(e.sp0, m0)) {
+        s0 = nextcoll(g);
+    } else {
+        g.skip();
+        s0 = 0.0;
+    }
+    s1 = 0.0;
+    if (e.ne == 2) {
+        if (stored(e.sp1, m1))
+            s1 = nextcoll(g);
+        else
+            g.skip();
+    }
 }
 
 }  // namespace pt
