@@ -536,7 +536,7 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const u
                         r = setr(Lq, kinenergy(qs, p), swarm, oob);
                     }
                     g.init(P.seed, uid, step);
-                    g.draw = d0;
+                    g.resume(d0);
                     iters = 0;
                     nev = 0;
                     active = true;
@@ -788,7 +788,7 @@ __global__ void k_rotate(const Params* __restrict__ Pp, int mark) {
 
 // push/select over the step's particles (first round) or the input work queues
 template <int BS, int SRC, bool EVENT, bool WALL = false>
-__global__ void __launch_bounds__(BS) k_advance(const Params* __restrict__ Pp) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) k_advance(const Params* __restrict__ Pp) {
     extern __shared__ double lds[];
     __shared__ Shared sh;
     __shared__ int skip;
@@ -890,7 +890,7 @@ __global__ void __launch_bounds__(BS) k_sample(const Params* __restrict__ Pp) {
             if (swarm < 0 || swarm >= P.nswarm) swarm = 0;
             sel = L.evp[idx];
             g.init(P.seed, uid, step);
-            g.draw = L.draw[idx];
+            g.resume(L.draw[idx]);
             E = kinenergy(sp, p);
             pn = norm3(p);
         }

@@ -116,20 +116,42 @@ struct PhiloxStream {
         draw = 0;
         cblk = 0;
     }
-    // advance past one draw whose value is not needed (the block is computed by the next
-    // draw that needs it)
-    __device__ __forceinline__ void skip() { draw++; }
+    // Invariant: when `draw` is odd, the block holding draw - 1 (and draw) is cached.  next()
+    // keeps it by itself; skip() and resume() compute the block when they leave `draw` odd.
+    __device__ __forceinline__ void fill(uint32_t blk) {
+        uint32_t a = u0, b = u1, c = step, e = blk;
+        philox10(a, b, c, e, k0, k1);
+        c0 = u01(a, b);
+        c1 = u01(c, e);
+        cblk = blk + 1;
+    }
+    // continue the stream at draw d0 (a particle resumed by another kernel)
+    __device__ __forceinline__ void resume(uint32_t d0) {
+        draw = d0;
+        if (d0 & 1) fill(d0 >> 1);
+    }
+    // advance past one draw whose value is not needed
+    __device__ __forceinline__ void skip() {
+        const uint32_t d = draw++;
+        if (!(d & 1) && cblk != (d >> 1) + 1) fill(d >> 1);
+    }
     __device__ __forceinline__ double next() {
         uint32_t d = draw++;
         uint32_t blk = d >> 1;
-        if (cblk != blk + 1) {
-            uint32_t a = u0, b = u1, c = step, e = blk;
-            philox10(a, b, c, e, k0, k1);
-            c0 = u01(a, b);
-            c1 = u01(c, e);
-            cblk = blk + 1;
-        }
+        if (cblk != blk + 1) fill(blk);
         return (d & 1) ? c1 : c0;
+    }
+    // the next two uniforms, as two next() calls return them, with at most one Philox block
+    // computed at one place: in a rejection loop every lane needs its block at the same
+    // instruction, instead of half of the lanes at each of two draw sites
+    __device__ __forceinline__ void next2(double& a, double& b) {
+        const uint32_t d = draw;
+        draw = d + 2;
+        const double first = c1;  // draw d when d is odd (the invariant)
+        const uint32_t need = (d + 1) >> 1;
+        if (cblk != need + 1) fill(need);
+        a = (d & 1) ? first : c0;
+        b = (d & 1) ? c0 : c1;
     }
 };
 
@@ -148,6 +170,10 @@ struct InjectedStream {
             return 0.5;
         }
         return u[pos++];
+    }
+    __device__ __forceinline__ void next2(double& a, double& b) {
+        a = next();
+        b = next();
     }
 };
 
@@ -291,9 +317,9 @@ template <class G>
 __device__ __forceinline__ double sample_rel_sr(double alpha, double beta, G& g) {
     double x;
     for (;;) {
-        double u = g.next();
+        double u, z;
+        g.next2(u, z);
         x = alpha * u / (alpha + 1.0 - u);
-        double z = g.next();
         if (z < 1.0 - beta * beta * x || g.exhausted()) break;
     }
     return 1.0 - 2.0 * x;
@@ -314,16 +340,21 @@ __device__ __forceinline__ double rbeb_sample(double T, double B, G& g) {
     double ratio = (t + 1.0) / (t - 1.0);
     double w;
     for (;;) {
-        double u = g.next();
+        double u, z;
+        g.next2(u, z);
         w = u / (ratio - u);
-        double w1 = 1.0 + w, tw = t - w;
-        double pb = pbn / (w1 * w1);
-        double wa = w + 1.0;
-        double g1 = 1.0 / wa + 1.0 / tw;
-        double g2 = 1.0 / (wa * wa) + 1.0 / (tw * tw);
-        double g3 = 1.0 / (wa * wa * wa) + 1.0 / (tw * tw * tw);
+        // the reference's 1/(1+w)^k and 1/(t-w)^k from two reciprocals (three divisions
+        // per trip instead of eight): g1 and w are exact, pb, g2 and g3 within a few ulps,
+        // so only a z * pb within ~1e-15 of p0 could be judged otherwise
+        double tw = t - w;
+        double ra = 1.0 / (1.0 + w), rt = 1.0 / tw;
+        double ra2 = ra * ra, rt2 = rt * rt;
+        double pb = pbn * ra2;
+        double g1 = ra + rt;
+        double g2 = ra2 + rt2;
+        double g3 = ra2 * ra + rt2 * rt;
         double p0 = A * g1 + Bc * (g2 + M) + Cc * g3;
-        if (g.next() * pb < p0 || g.exhausted()) break;
+        if (z * pb < p0 || g.exhausted()) break;
     }
     return B * w;
 }
@@ -339,11 +370,12 @@ __device__ __forceinline__ double moller_sample(double tcut, double eng, G& g) {
     double cc = gm * gm;
     double eps;
     for (;;) {
-        double r = g.next();
+        double r, z;
+        g.next2(r, z);
         eps = e0 / (1.0 - r + 2.0 * e0 * r);
         double om = 1.0 - eps;
         double gg = pre * (ca * (eps * eps) - cb * (eps / om) + cc / (om * om));
-        if (g.next() < gg || g.exhausted()) break;
+        if (z < gg || g.exhausted()) break;
     }
     return eps * eng;
 }
@@ -370,12 +402,13 @@ __device__ __forceinline__ double bhabha_sample(double tcut, double eng, G& g) {
     double g2 = (((B0 + B1 * e0) + B2 * (e0 * e0)) + B3 * (e0 * e0 * e0)) + B4 * pow(e0, 4.0);
     double eps;
     for (;;) {
-        double r = g.next();
+        double r, z;
+        g.next2(r, z);
         eps = e0 / (1.0 - r + e0 * r);
         double g1 = (((B0 + B1 * eps) + B2 * (eps * eps)) + B3 * (eps * eps * eps)) +
                     B4 * pow(eps, 4.0);
         double gg = g1 / g2;
-        if (g.next() < gg || g.exhausted()) break;
+        if (z < gg || g.exhausted()) break;
     }
     return eps * eng;
 }
@@ -546,9 +579,11 @@ __device__ __forceinline__ double annih_sample(double eng, G& g) {
     double den = (gm + 1.0) * (gm + 1.0);
     double eps;
     for (;;) {
-        eps = emin * pow(ratio, g.next());
+        double r, z;
+        g.next2(r, z);
+        eps = emin * pow(ratio, r);
         double gg = 1.0 - eps + (2.0 * gm * eps - 1.0) / (eps * den);
-        if (g.next() < gg || g.exhausted()) break;
+        if (z < gg || g.exhausted()) break;
     }
     return eps;
 }
