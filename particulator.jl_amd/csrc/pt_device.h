@@ -201,6 +201,17 @@ __device__ __forceinline__ double dot3(const V3& a, const V3& b) {
 }
 __device__ __forceinline__ double norm3(const V3& a) { return sqrt(dot3(a, a)); }
 
+// a / b for a constant b with inv = 1/b correctly rounded: the product a * inv is within an
+// ulp of a/b, the remainder a - q b is exact in one FMA, and one more FMA rounds the quotient
+// correctly (Markstein, IBM J. Res. Dev. 34 (1990) 111-119), so the result is a / b bit for
+// bit (normal operands; NaN and infinity pass through) in three operations instead of a
+// full division
+__device__ __forceinline__ double div_const(double a, double b, double inv) {
+    const double q = a * inv;
+    const double r = fma(-q, b, a);
+    return (r == 0.0 || r != r) ? q : fma(r, inv, q);
+}
+
 // src/electron.jl:53, src/photon.jl:52
 __device__ __forceinline__ double kinenergy(int sp, const V3& p) {
     double pp = dot3(p, p);
@@ -219,7 +230,8 @@ __device__ __forceinline__ V3 velocity(int sp, const V3& p) {
         double f = C / norm3(p);
         return V3{p.x * f, p.y * f, p.z * f};
     }
-    double g = sqrt(1.0 + C2 * dot3(p, p) / (MC2 * MC2));
+    constexpr double MC2SQ = MC2 * MC2;
+    double g = sqrt(1.0 + div_const(C2 * dot3(p, p), MC2SQ, 1.0 / MC2SQ));
     double inv = 1.0 / (ME * g);
     return V3{p.x * inv, p.y * inv, p.z * inv};
 }
@@ -338,16 +350,19 @@ __device__ __forceinline__ double rbeb_sample(double T, double B, G& g) {
     double M = b1 * b1 / ((1.0 + t1) * (1.0 + t1));
     double pbn = 2.0 * Bc + 2.0 * Cc + (t + 1.0) * (t + 1.0) * Bc * M / 4.0;
     double ratio = (t + 1.0) / (t - 1.0);
+    const double iratio = 1.0 / ratio;
     double w;
     for (;;) {
         double u, z;
         g.next2(u, z);
-        w = u / (ratio - u);
-        // the reference's 1/(1+w)^k and 1/(t-w)^k from two reciprocals (three divisions
-        // per trip instead of eight): g1 and w are exact, pb, g2 and g3 within a few ulps,
-        // so only a z * pb within ~1e-15 of p0 could be judged otherwise
+        const double den = ratio - u;
+        w = u / den;
+        // the reference's 1/(1+w)^k and 1/(t-w)^k from 1/(1+w) = (ratio - u) / ratio and one
+        // reciprocal (two divisions per trip instead of eight): w is exact, the acceptance
+        // terms within a few ulps, so only a z * pb within ~1e-15 of p0 could be judged
+        // otherwise
         double tw = t - w;
-        double ra = 1.0 / (1.0 + w), rt = 1.0 / tw;
+        double ra = den * iratio, rt = 1.0 / tw;
         double ra2 = ra * ra, rt2 = rt * rt;
         double pb = pbn * ra2;
         double g1 = ra + rt;
@@ -654,8 +669,11 @@ __device__ __forceinline__ void push(int sp, const Field& f, V3& x, V3& p, doubl
     V3 v1 = velocity(sp, p);
     V3 f1 = force(sp, f, x, p);
     double dt2 = 2.0 * dt;
-    V3 x2{x.x + dt2 * v1.x / 3.0, x.y + dt2 * v1.y / 3.0, x.z + dt2 * v1.z / 3.0};
-    V3 p2{p.x + dt2 * f1.x / 3.0, p.y + dt2 * f1.y / 3.0, p.z + dt2 * f1.z / 3.0};
+    constexpr double THIRD = 1.0 / 3.0;
+    V3 x2{x.x + div_const(dt2 * v1.x, 3.0, THIRD), x.y + div_const(dt2 * v1.y, 3.0, THIRD),
+          x.z + div_const(dt2 * v1.z, 3.0, THIRD)};
+    V3 p2{p.x + div_const(dt2 * f1.x, 3.0, THIRD), p.y + div_const(dt2 * f1.y, 3.0, THIRD),
+          p.z + div_const(dt2 * f1.z, 3.0, THIRD)};
     V3 v2 = velocity(sp, p2);
     V3 f2 = force(sp, f, x2, p2);
     x.x = x.x + dt * (v1.x / 4.0 + 3.0 * v2.x / 4.0);
