@@ -150,7 +150,7 @@ struct SpLite {
     ull cap;
     const double* rate;  // LDS
     const double* rb;    // LDS (one swarm) or global
-    double cut, xmax;
+    double cut, xmax, ixmax;  // ixmax = 1 / xmax
     int tracked, valid, nprocs, k, nswarm, pad;
 };
 
@@ -196,6 +196,7 @@ __device__ __forceinline__ void stage(const Params& P, double* lds, Shared& sh) 
             L.nswarm = tb.nswarm;
             L.cut = tb.cut;
             L.xmax = tb.xmax;
+            L.ixmax = 1.0 / tb.xmax;
             L.rate = lds + P.lds_rate_off[s];
             L.rb = P.rb_in_lds ? lds + P.lds_rb_off[s] : tb.ratebound;
             sh.cut[s] = tb.cut;
@@ -220,7 +221,7 @@ __device__ __forceinline__ double setr(const SpLite& L, double E, int swarm, boo
     if (E < L.cut) return 0.0;
     double t1, t2;
     bool o;
-    int i = precheb(E, L.k, L.xmax, t1, t2, o);
+    int i = precheb(E, L.k, L.xmax, L.ixmax, t1, t2, o);
     oob |= o;
     const double* rb = L.rb + (size_t)(L.nswarm > 1 ? swarm : 0) * (L.k + 1) * 3 + i * 3;
     return cheb3(rb, t1, t2);
@@ -585,7 +586,7 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const u
                             nev++;
                             double t1, t2;
                             bool o;
-                            int iv = precheb(E, L.k, L.xmax, t1, t2, o);
+                            int iv = precheb(E, L.k, L.xmax, L.ixmax, t1, t2, o);
                             oob |= o;
                             double xi = g.next() * r;
                             const double* rt = L.rate + (size_t)iv * L.nprocs * 3;
@@ -1727,7 +1728,7 @@ __global__ void __launch_bounds__(256) k_dbg_select(const Params* __restrict__ P
     double r = setr(L, E[i], sw, oob);
     double t1, t2;
     bool o;
-    int iv = precheb(E[i], L.k, L.xmax, t1, t2, o);
+    int iv = precheb(E[i], L.k, L.xmax, L.ixmax, t1, t2, o);
     double xi = u[i] * r;
     const double* rt = L.rate + (size_t)iv * L.nprocs * 3;
     int sel = -1;
@@ -3842,6 +3843,7 @@ int pt_set_continuum_loss(pt_ctx* c, int nintervals, int order, double xmax, con
         cl.c[0] = d[0];
         cl.c[1] = d[1];
         cl.xmax = xmax;
+        cl.ixmax = 1.0 / xmax;
         cl.k = nintervals;
         cl.order = order;
         cl.on = 1;

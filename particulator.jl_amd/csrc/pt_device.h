@@ -61,7 +61,7 @@ struct FieldFn {
 // [(k+1)][order] (the reference's (order, k+1) column-major)
 struct ContLoss {
     const double* c[2];  // electron, positron
-    double xmax;
+    double xmax, ixmax;  // ixmax = 1 / xmax
     int k, order, on, pad;
 };
 
@@ -239,9 +239,9 @@ __device__ __forceinline__ V3 velocity(int sp, const V3& p) {
 // ---------------------------------------------------------------- Chebyshev
 // precheb, src/cheby.jl:127-143 (+ chebval :57-81).  The index is clamped to [0, k]
 // (the reference reads out of bounds above xmax); `oob` reports it.
-__device__ __forceinline__ int precheb(double x, int k, double xmax, double& t1, double& t2,
-                                       bool& oob) {
-    double x1 = x / xmax;
+__device__ __forceinline__ int precheb(double x, int k, double xmax, double ixmax, double& t1,
+                                       double& t2, bool& oob) {
+    double x1 = div_const(x, xmax, ixmax);  // x / xmax; ixmax = 1 / xmax
     int l = 0;
     double s;
     if (x1 != x1) {
@@ -628,10 +628,11 @@ __device__ __forceinline__ V3 field_eval(const FieldFn& f, const V3& x) {
 // force(::ElectromagneticField, s), src/field.jl:62-68
 // chebeval with `order` terms (src/cheby.jl:55-81, :111-143): T_j = (2 xi) T_{j-1} - T_{j-2},
 // summed left to right; the interval index is clamped to [0, k]
-__device__ __forceinline__ double chebeval_n(const double* c, int k, int order, double xmax, double x) {
+__device__ __forceinline__ double chebeval_n(const double* c, int k, int order, double xmax, double ixmax,
+                                             double x) {
     double t1, t2;
     bool oob;
-    int i = precheb(x, k, xmax, t1, t2, oob);
+    int i = precheb(x, k, xmax, ixmax, t1, t2, oob);
     const double* a = c + (size_t)i * order;
     double tm2 = 1.0, tm1 = t1;
     double acc = a[0] * 1.0;
@@ -657,7 +658,8 @@ __device__ __forceinline__ V3 force(int sp, const Field& f, const V3& x, const V
     if (f.cl.on) {
         // CombinedForcing(field, ContinuumLoss): the sum of the forces (src/pusher.jl:14-23);
         // the loss term is p * (-f / |p|) (src/continuum.jl:17-20)
-        double fc = chebeval_n(f.cl.c[sp == PT_ELECTRON ? 0 : 1], f.cl.k, f.cl.order, f.cl.xmax, kinenergy(sp, p));
+        double fc = chebeval_n(f.cl.c[sp == PT_ELECTRON ? 0 : 1], f.cl.k, f.cl.order, f.cl.xmax, f.cl.ixmax,
+                                kinenergy(sp, p));
         double sc = -fc / norm3(p);
         F = V3{F.x + p.x * sc, F.y + p.y * sc, F.z + p.z * sc};
     }
