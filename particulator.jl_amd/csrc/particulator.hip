@@ -129,6 +129,7 @@ struct Params {
     MlpDev mlp[NLEARN];   // learned samplers (bremsstrahlung, Moller) used by the sampling kernel
     WallDev wall;         // WallCallback per species
     int lds_mlp_off;      // byte offset of the staged network in the sampling kernel's LDS
+    int lds_coop_off;     // byte offset of the step kernels' per-wave RBEB job area
     int lds_rate_off[PT_NSPECIES];
     int lds_rb_off[PT_NSPECIES];
     int lds_doubles;
@@ -382,6 +383,97 @@ __device__ __forceinline__ void wave_push(ull* cnt, uint32_t* arr, uint32_t val,
     }
 }
 
+// A lane's RBEB collision waiting for the wave's cooperative rejection loop (LDS, one slot
+// per lane of the wave, then a 64-entry rank table)
+struct RbebJob {
+    RbebSetup s;
+    double first;  // uniform number d0 of the stream when d0 is odd (a cached block's half)
+    uint64_t uid;
+    uint32_t d0, k;  // stream position before the loop, trips evaluated so far
+};
+constexpr int COOP_WAVE_BYTES = 64 * (int)sizeof(RbebJob) + 64;
+
+// The rejection loops of the wave's pending RBEB collisions (_rbeb_sample,
+// src/rbeb.jl:170-196), run by every lane of the wave together.  Trip k of a particle uses
+// draws d0 + 2k and d0 + 2k + 1 of its stream, so any lane can evaluate it: each round the
+// lanes are shared out over the particles still waiting (64 / n trips each, in order), and
+// a particle takes its first accepting trip.  Result and stream position are those of the
+// sequential loop; the wave runs about log(n) rounds instead of the longest particle's
+// trip count.  Called by the whole wave; `pending` lanes have written their job.
+__device__ __forceinline__ void rbeb_coop(const Params& P, char* wlds, int lane, uint64_t ltmask, uint32_t step,
+                                          bool& pending, double& wout, PhiloxStream& g) {
+    RbebJob* jobs = (RbebJob*)wlds;
+    uint8_t* rank_lane = (uint8_t*)(wlds + 64 * sizeof(RbebJob));
+    const uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
+    uint64_t U = __ballot(pending);
+    while (U) {
+        const int nU = __popcll(U);
+        const int m = 64 / nU;
+        const int myrank = __popcll(U & ltmask);
+        if (pending) rank_lane[myrank] = (uint8_t)lane;
+        wave_lds_order();
+        const int gi = lane / m, j = lane - gi * m;
+        const bool act = gi < nU;
+        double c0b = 0.0, c1b = 0.0, wv = 0.0;
+        uint32_t blk = 0, D = 0;
+        int ow = 0;
+        if (act) {
+            ow = rank_lane[gi];
+            const RbebJob& J = jobs[ow];
+            D = J.d0 + 2u * (J.k + (uint32_t)j);
+            blk = (D + 1) >> 1;  // the block of the trip's second draw
+            uint32_t a = (uint32_t)J.uid, b = (uint32_t)(J.uid >> 32), c = step, e = blk;
+            philox10(a, b, c, e, k0, k1);
+            c0b = u01(a, b);
+            c1b = u01(c, e);
+        }
+        const double prev = __shfl_up(c1b, 1);  // the previous trip's block (same particle when j > 0)
+        bool acc = false;
+        if (act) {
+            const RbebJob& J = jobs[ow];
+            double u, z;
+            if (D & 1) {
+                u = j == 0 ? J.first : prev;
+                z = c0b;
+            } else {
+                u = c0b;
+                z = c1b;
+            }
+            acc = rbeb_trip(J.s, u, z, wv) || D + 2u >= DRAW_LIMIT;
+        }
+        const uint64_t Acc = __ballot(act && acc);
+        int src = lane, last = lane;
+        bool got = false;
+        if (pending) {
+            const uint64_t gm = m >= 64 ? ~0ull : ((1ull << m) - 1ull);
+            const uint64_t bits = (Acc >> (myrank * m)) & gm;
+            got = bits != 0;
+            if (got) src = myrank * m + __ffsll((unsigned long long)bits) - 1;
+            last = myrank * m + m - 1;
+        }
+        const double w_s = __shfl(wv, src), c0_s = __shfl(c0b, src), c1_s = __shfl(c1b, src);
+        const uint32_t blk_s = __shfl(blk, src), D_s = __shfl(D, src);
+        const double c1_last = __shfl(c1b, last);
+        wave_lds_order();
+        if (pending) {
+            RbebJob& J = jobs[lane];
+            if (got) {
+                wout = w_s;
+                g.draw = D_s + 2u;
+                g.c0 = c0_s;
+                g.c1 = c1_s;
+                g.cblk = blk_s + 1;
+                pending = false;
+            } else {
+                J.k += (uint32_t)m;
+                J.first = c1_last;
+            }
+        }
+        wave_lds_order();
+        U = __ballot(pending);
+    }
+}
+
 // onadvance of WallCallback (src/callback.jl:166-182): an upward crossing of x[coord] = v
 // records lincomb(new, old, w) (src/electron.jl:127-132), w = (v - old) / (new - old); the
 // particle is dropped if the wall says so.  The recorded state's s and r are not drawn.
@@ -414,7 +506,7 @@ __device__ __forceinline__ void wall_check(const Params& P, int sp, const V3& xo
 }
 
 template <int SRC, bool EVENT, bool WALL = false>
-__device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const ull* rb, const ull* re,
+__device__ __forceinline__ void select_loop(const Params& P, Shared& sh, char* wlds, const ull* rb, const ull* re,
                                             ull* cursor, double tfinal, uint32_t step, bool main_pass,
                                             int par) {
     const ull n0 = re[0] - rb[0], n1 = re[1] - rb[1], n2 = re[2] - rb[2];
@@ -549,113 +641,136 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, const u
         }
         // 3. the wave leaves when its items are claimed and no lane holds a particle
         if (drained && __ballot(have) == 0) break;
-        // 4. one push (and at most one collision) of this lane's particle
+        // 4. one push of this lane's particle and, at the end of it, the selection; a real
+        //    collision is finished in 6, after the wave's RBEB rejection loops (5)
         ull* qc = nullptr;
         uint32_t* qa = nullptr;
         uint32_t qv = 0;
-        if (have) {
+        int qsel = -1;          // collision queued for the sampling kernel
+        int csel = -1;          // collision sampled in 6
+        double cE = 0.0;        // its kinetic energy
+        bool rb_wait = false;   // an RBEB collision waiting for the cooperative loop
+        double rb_w = -1.0;
+        if (have && trem > EPS64 && active) {
             const SpLite& L = sh.sp[sp];
-            int qsel = -1;
-            if (trem > EPS64 && active) {
-                if (++iters > ITER_LIMIT || g.exhausted()) {
-                    atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
-                    trem = 0.0;
+            if (++iters > ITER_LIMIT || g.exhausted()) {
+                atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
+                trem = 0.0;
+            } else {
+                double tnext = s / r;
+                double dt;
+                bool coll;
+                if (trem > tnext) {
+                    dt = tnext;
+                    coll = true;
                 } else {
-                    double tnext = s / r;
-                    double dt;
-                    bool coll;
-                    if (trem > tnext) {
-                        dt = tnext;
-                        coll = true;
-                    } else {
-                        dt = trem;
-                        coll = false;
-                        s -= dt * r;
-                    }
-                    if constexpr (WALL) {
-                        const V3 xo = x, po = p;
-                        const double to = t;
-                        push(sp, P.fields[swarm], x, p, t, dt);
-                        wall_check(P, sp, xo, po, to, x, p, w, t, uid, swarm, active);
-                    } else {
-                        push(sp, P.fields[swarm], x, p, t, dt);
-                    }
-                    if (coll && r != 0.0 && (!WALL || active)) {
-                        double E = kinenergy(sp, p);
-                        if (E >= L.cut) {
-                            nev++;
-                            double t1, t2;
-                            bool o;
-                            int iv = precheb(E, L.k, L.xmax, L.ixmax, t1, t2, o);
-                            oob |= o;
-                            double xi = g.next() * r;
-                            const double* rt = L.rate + (size_t)iv * L.nprocs * 3;
-                            int sel = -1;
-                            for (int j = 0; j < L.nprocs; j++) {
-                                double nu = cheb3(rt + 3 * j, t1, t2);
-                                if (nu > xi) {
-                                    sel = j;
-                                    break;
-                                }
-                                xi -= nu;
+                    dt = trem;
+                    coll = false;
+                    s -= dt * r;
+                }
+                if constexpr (WALL) {
+                    const V3 xo = x, po = p;
+                    const double to = t;
+                    push(sp, P.fields[swarm], x, p, t, dt);
+                    wall_check(P, sp, xo, po, to, x, p, w, t, uid, swarm, active);
+                } else {
+                    push(sp, P.fields[swarm], x, p, t, dt);
+                }
+                if (coll && r != 0.0 && (!WALL || active)) {
+                    double E = kinenergy(sp, p);
+                    if (E >= L.cut) {
+                        nev++;
+                        double t1, t2;
+                        bool o;
+                        int iv = precheb(E, L.k, L.xmax, L.ixmax, t1, t2, o);
+                        oob |= o;
+                        double xi = g.next() * r;
+                        const double* rt = L.rate + (size_t)iv * L.nprocs * 3;
+                        int sel = -1;
+                        for (int j = 0; j < L.nprocs; j++) {
+                            double nu = cheb3(rt + 3 * j, t1, t2);
+                            if (nu > xi) {
+                                sel = j;
+                                break;
                             }
-                            if (sel < 0) {
-                                if (!(xi >= 0.0)) atomicAdd(&sh.bc.flags[FLAG_RATEBOUND], 1ull);
-                                atomicAdd(&sh.bc.counts[sp * NCOUNT], 1ull);
-                                r = setr(L, E, swarm, oob);
-                                s = nextcoll(g);
+                            xi -= nu;
+                        }
+                        if (sel < 0) {
+                            if (!(xi >= 0.0)) atomicAdd(&sh.bc.flags[FLAG_RATEBOUND], 1ull);
+                            atomicAdd(&sh.bc.counts[sp * NCOUNT], 1ull);
+                            r = setr(L, E, swarm, oob);
+                            s = nextcoll(g);
+                        } else {
+                            atomicAdd(&sh.bc.counts[sp * NCOUNT + 1 + sel], 1ull);
+                            const DevProc& pr = sh.procs[sp][sel];
+                            if (EVENT && ((P.defer_mask >> pr.kind) & 1u)) {
+                                qsel = sel;
                             } else {
-                                atomicAdd(&sh.bc.counts[sp * NCOUNT + 1 + sel], 1ull);
-                                if (EVENT && ((P.defer_mask >> sh.procs[sp][sel].kind) & 1u)) {
-                                    qsel = sel;
-                                } else {
-                                    double pn = norm3(p);
-                                    Ev e;
-                                    sample_event(sh.procs[sp][sel], sp, p, pn, E, P.sb, P.pe, g, e);
-                                    if (e.err) atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
-                                    if (e.dep != 0.0) deposit(P, sh.bc, e.dep * w, x.z);
-                                    V3 m0, m1;
-                                    double s0, s1;
-                                    finish_event(p, pn, e, g, m0, m1, s0, s1, sh.cut, sh.stored);
-                                    uint32_t dend = g.draw;
-                                    if (e.keep) {
-                                        p = m0;
-                                        s = s0;
-                                        r = setr(L, kinenergy(sp, p), swarm, oob);
-                                    } else {
-                                        active = false;
-                                        ull sl = append(P, sh, e.sp0, x, m0, w, t, s0, uid, step, dend, 0,
-                                                        swarm, main_pass, tfinal);
-                                        if (sl != NO_SLOT && nch < NCHILD) {
-                                            if (nch == 0) {
-                                                chsp[0] = e.sp0;
-                                                chidx[0] = sl;
-                                            } else {
-                                                chsp[1] = e.sp0;
-                                                chidx[1] = sl;
-                                            }
-                                            nch++;
-                                        }
-                                    }
-                                    if (e.ne == 2) {
-                                        ull sl = append(P, sh, e.sp1, x, m1, w, t, s1, uid, step, dend,
-                                                        e.keep ? 0 : 1, swarm, main_pass, tfinal);
-                                        if (sl != NO_SLOT && nch < NCHILD) {
-                                            if (nch == 0) {
-                                                chsp[0] = e.sp1;
-                                                chidx[0] = sl;
-                                            } else {
-                                                chsp[1] = e.sp1;
-                                                chidx[1] = sl;
-                                            }
-                                            nch++;
-                                        }
-                                    }
+                                csel = sel;
+                                cE = E;
+                                if (pr.kind == PT_PROC_RBEB) {
+                                    RbebJob& J = ((RbebJob*)wlds)[lane];
+                                    J.s = rbeb_setup(E, pr.B);
+                                    J.first = g.c1;
+                                    J.uid = uid;
+                                    J.d0 = g.draw;
+                                    J.k = 0;
+                                    rb_wait = true;
                                 }
                             }
                         }
                     }
-                    trem -= dt;
+                }
+                trem -= dt;
+            }
+        }
+        // 5. every lane of the wave runs the pending RBEB rejection loops
+        rbeb_coop(P, wlds, lane, ltmask, step, rb_wait, rb_w, g);
+        // 6. the sampled collision, then the end of the particle's step
+        if (have) {
+            const SpLite& L = sh.sp[sp];
+            if (csel >= 0) {
+                const double E = cE;
+                double pn = norm3(p);
+                Ev e;
+                sample_event(sh.procs[sp][csel], sp, p, pn, E, P.sb, P.pe, g, e, rb_w);
+                if (e.err) atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
+                if (e.dep != 0.0) deposit(P, sh.bc, e.dep * w, x.z);
+                V3 m0, m1;
+                double s0, s1;
+                finish_event(p, pn, e, g, m0, m1, s0, s1, sh.cut, sh.stored);
+                uint32_t dend = g.draw;
+                if (e.keep) {
+                    p = m0;
+                    s = s0;
+                    r = setr(L, kinenergy(sp, p), swarm, oob);
+                } else {
+                    active = false;
+                    ull sl = append(P, sh, e.sp0, x, m0, w, t, s0, uid, step, dend, 0, swarm, main_pass, tfinal);
+                    if (sl != NO_SLOT && nch < NCHILD) {
+                        if (nch == 0) {
+                            chsp[0] = e.sp0;
+                            chidx[0] = sl;
+                        } else {
+                            chsp[1] = e.sp0;
+                            chidx[1] = sl;
+                        }
+                        nch++;
+                    }
+                }
+                if (e.ne == 2) {
+                    ull sl = append(P, sh, e.sp1, x, m1, w, t, s1, uid, step, dend, e.keep ? 0 : 1, swarm, main_pass,
+                                    tfinal);
+                    if (sl != NO_SLOT && nch < NCHILD) {
+                        if (nch == 0) {
+                            chsp[0] = e.sp1;
+                            chidx[0] = sl;
+                        } else {
+                            chsp[1] = e.sp1;
+                            chidx[1] = sl;
+                        }
+                        nch++;
+                    }
                 }
             }
             if (EVENT && qsel >= 0) {
@@ -814,7 +929,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) k_
     __syncthreads();
     if (skip) return;
     stage(P, lds, sh);
-    select_loop<SRC, EVENT, WALL>(P, sh, rb, re, &P.clock->cursor, C.tfinal, C.step,
+    char* wlds = (char*)lds + P.lds_coop_off + (threadIdx.x >> 6) * COOP_WAVE_BYTES;
+    select_loop<SRC, EVENT, WALL>(P, sh, wlds, rb, re, &P.clock->cursor, C.tfinal, C.step,
                                   SRC == SRC_RANGE && C.pass == 0, par);
     flush_counters(P, &sh.bc);
 }
@@ -1004,7 +1120,8 @@ __global__ void __launch_bounds__(512) k_tail(const Params* __restrict__ Pp, int
             rb[s] = beg[s];
             re[s] = end[s];
         }
-        select_loop<SRC_RANGE, false, WALL>(P, sh, rb, re, &cur, tfinal, step, false, 0);
+        select_loop<SRC_RANGE, false, WALL>(P, sh, (char*)lds + P.lds_coop_off + (threadIdx.x >> 6) * COOP_WAVE_BYTES,
+                                            rb, re, &cur, tfinal, step, false, 0);
         __threadfence();
         __syncthreads();
         if (threadIdx.x == 0)
@@ -1943,6 +2060,7 @@ static int sync_params(pt_ctx* c) {
     c->hp.lds_doubles = off;
     c->lds_bytes = off * (int)sizeof(double);
     c->hp.lds_mlp_off = (c->lds_bytes + 15) & ~15;
+    c->hp.lds_coop_off = c->hp.lds_mlp_off;
     HIPCHK(c, hipMemcpyAsync(c->dp, &c->hp, sizeof(Params), hipMemcpyHostToDevice, c->stream));
     c->dirty = false;
     return PT_OK;
@@ -2511,8 +2629,10 @@ static bool mlp_any(const pt_ctx* c) { return c->hp.mlp[0].on || c->hp.mlp[1].on
 // dynamic LDS of a kernel variant: the rate tables, and for the sampling kernel with a
 // learned sampler in use, the per-wave scratch of the network
 static size_t kernel_lds(const pt_ctx* c, int v, int bs) {
-    if (v == KV_SAMPLE && mlp_any(c)) return (size_t)c->hp.lds_mlp_off + (size_t)(bs / 64) * MLP_WAVE_BYTES;
-    return (size_t)c->lds_bytes;
+    if (v == KV_SAMPLE)
+        return mlp_any(c) ? (size_t)c->hp.lds_mlp_off + (size_t)(bs / 64) * MLP_WAVE_BYTES : (size_t)c->lds_bytes;
+    // the advance kernels: tables, then the per-wave RBEB job areas
+    return (size_t)c->hp.lds_coop_off + (size_t)(bs / 64) * COOP_WAVE_BYTES;
 }
 
 static int ensure_residency(pt_ctx* c) {
@@ -2602,8 +2722,9 @@ static int launch_step(pt_ctx* c, bool clocked, double tfinal) {
         if ((rc = launch_kernel_args(c, wall ? KV_RANGE_WALL : KV_RANGE_FULL, true))) return rc;
     }
     if (prof && (rc = mark(c, &e1))) return rc;
-    if (wall) hipLaunchKernelGGL(k_tail<true>, dim3(1), dim3(512), c->lds_bytes, c->stream, c->dp, 100000);
-    else hipLaunchKernelGGL(k_tail<false>, dim3(1), dim3(512), c->lds_bytes, c->stream, c->dp, 100000);
+    const size_t tail_lds = (size_t)c->hp.lds_coop_off + (size_t)(512 / 64) * COOP_WAVE_BYTES;
+    if (wall) hipLaunchKernelGGL(k_tail<true>, dim3(1), dim3(512), tail_lds, c->stream, c->dp, 100000);
+    else hipLaunchKernelGGL(k_tail<false>, dim3(1), dim3(512), tail_lds, c->stream, c->dp, 100000);
     HIPCHK(c, hipGetLastError());
     if (prof) {
         if ((rc = mark(c, &e2))) return rc;

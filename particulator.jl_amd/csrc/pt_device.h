@@ -337,39 +337,51 @@ __device__ __forceinline__ double sample_rel_sr(double alpha, double beta, G& g)
     return 1.0 - 2.0 * x;
 }
 
-// rbeb_sample / _rbeb_sample, src/rbeb.jl:156-196
-template <class G>
-__device__ __forceinline__ double rbeb_sample(double T, double B, G& g) {
+// rbeb_sample / _rbeb_sample, src/rbeb.jl:156-196, in three parts: the constants of the
+// rejection loop, one trip (uniforms u, z), and the loop.  A trip is a function of the two
+// uniforms alone, so the wave can also run the trips of its particles in parallel.
+struct RbebSetup {
+    double t, A, Cc, M, pbn, ratio, iratio;
+};
+__device__ __forceinline__ RbebSetup rbeb_setup(double T, double B) {
+    RbebSetup s;
     double t1 = T / (ME * C2);
     double b1 = B / (ME * C2);
     double bt2 = 1.0 - 1.0 / ((1.0 + t1) * (1.0 + t1));
-    double t = T / B;
-    double A = -(1.0 + 2.0 * t1) / (t + 1.0) / ((1.0 + t1) * (1.0 + t1));
-    const double Bc = 1.0;
-    double Cc = (log(bt2 / (1.0 - bt2)) - bt2) - log(2.0 * b1);
-    double M = b1 * b1 / ((1.0 + t1) * (1.0 + t1));
-    double pbn = 2.0 * Bc + 2.0 * Cc + (t + 1.0) * (t + 1.0) * Bc * M / 4.0;
-    double ratio = (t + 1.0) / (t - 1.0);
-    const double iratio = 1.0 / ratio;
+    s.t = T / B;
+    s.A = -(1.0 + 2.0 * t1) / (s.t + 1.0) / ((1.0 + t1) * (1.0 + t1));
+    s.Cc = (log(bt2 / (1.0 - bt2)) - bt2) - log(2.0 * b1);
+    s.M = b1 * b1 / ((1.0 + t1) * (1.0 + t1));
+    s.pbn = 2.0 * 1.0 + 2.0 * s.Cc + (s.t + 1.0) * (s.t + 1.0) * 1.0 * s.M / 4.0;
+    s.ratio = (s.t + 1.0) / (s.t - 1.0);
+    s.iratio = 1.0 / s.ratio;
+    return s;
+}
+// One trip: w (exact) and whether it is accepted.  The reference's 1/(1+w)^k and
+// 1/(t-w)^k come from 1/(1+w) = (ratio - u) / ratio and one reciprocal (two divisions
+// instead of eight): the acceptance terms are within a few ulps of the reference's, so
+// only a z * pb within ~1e-15 of p0 could be judged otherwise.
+__device__ __forceinline__ bool rbeb_trip(const RbebSetup& s, double u, double z, double& w) {
+    const double den = s.ratio - u;
+    w = u / den;
+    double tw = s.t - w;
+    double ra = den * s.iratio, rt = 1.0 / tw;
+    double ra2 = ra * ra, rt2 = rt * rt;
+    double pb = s.pbn * ra2;
+    double g1 = ra + rt;
+    double g2 = ra2 + rt2;
+    double g3 = ra2 * ra + rt2 * rt;
+    double p0 = s.A * g1 + 1.0 * (g2 + s.M) + s.Cc * g3;
+    return z * pb < p0;
+}
+template <class G>
+__device__ __forceinline__ double rbeb_sample(double T, double B, G& g) {
+    const RbebSetup s = rbeb_setup(T, B);
     double w;
     for (;;) {
         double u, z;
         g.next2(u, z);
-        const double den = ratio - u;
-        w = u / den;
-        // the reference's 1/(1+w)^k and 1/(t-w)^k from 1/(1+w) = (ratio - u) / ratio and one
-        // reciprocal (two divisions per trip instead of eight): w is exact, the acceptance
-        // terms within a few ulps, so only a z * pb within ~1e-15 of p0 could be judged
-        // otherwise
-        double tw = t - w;
-        double ra = den * iratio, rt = 1.0 / tw;
-        double ra2 = ra * ra, rt2 = rt * rt;
-        double pb = pbn * ra2;
-        double g1 = ra + rt;
-        double g2 = ra2 + rt2;
-        double g3 = ra2 * ra + rt2 * rt;
-        double p0 = A * g1 + Bc * (g2 + M) + Cc * g3;
-        if (z * pb < p0 || g.exhausted()) break;
+        if (rbeb_trip(s, u, z, w) || g.exhausted()) break;
     }
     return B * w;
 }
@@ -711,7 +723,7 @@ struct Ev {
 template <class G>
 __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3& p, double pnorm,
                                              double eng, const DevSB* sbs, const DevPE* pes, G& g,
-                                             Ev& e) {
+                                             Ev& e, double rbeb_w = -1.0) {
     e.err = 0;
     e.nturn = 1;
     e.sgnB = 1;
@@ -740,7 +752,8 @@ __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3
     case PT_PROC_BHABHA: {  // src/bhaba.jl:9-33
         double E0 = eng, E1, E2;
         if (pr.kind == PT_PROC_RBEB) {
-            E2 = rbeb_sample(E0, pr.B, g);
+            // rbeb_w >= 0: the rejection loop already ran (its stream position included)
+            E2 = rbeb_w >= 0.0 ? pr.B * rbeb_w : rbeb_sample(E0, pr.B, g);
             E1 = E0 - E2 - pr.B;
             e.dep = pr.B;
             if (!(E2 < E1)) e.err = 1;
