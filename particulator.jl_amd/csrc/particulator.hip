@@ -732,8 +732,12 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, char* w
             if (csel >= 0) {
                 const double E = cE;
                 double pn = norm3(p);
+                // the two common kinds draw their azimuth first here: one draw site for both
+                const int kind = sh.procs[sp][csel].kind;
+                double phi = -1.0;
+                if (kind == PT_PROC_ELASTIC || (kind == PT_PROC_RBEB && rb_w >= 0.0)) phi = TWO_PI * g.next();
                 Ev e;
-                sample_event(sh.procs[sp][csel], sp, p, pn, E, P.sb, P.pe, g, e, rb_w);
+                sample_event(sh.procs[sp][csel], sp, p, pn, E, P.sb, P.pe, g, e, rb_w, phi);
                 if (e.err) atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
                 if (e.dep != 0.0) deposit(P, sh.bc, e.dep * w, x.z);
                 V3 m0, m1;

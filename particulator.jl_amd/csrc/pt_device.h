@@ -723,7 +723,9 @@ struct Ev {
 template <class G>
 __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3& p, double pnorm,
                                              double eng, const DevSB* sbs, const DevPE* pes, G& g,
-                                             Ev& e, double rbeb_w = -1.0) {
+                                             Ev& e, double rbeb_w = -1.0, double phi = -1.0) {
+    // phi >= 0: the azimuth was drawn by the caller at the point where this event draws
+    // it (elastic: first; RBEB after a finished rejection loop: first)
     e.err = 0;
     e.nturn = 1;
     e.sgnB = 1;
@@ -734,7 +736,7 @@ __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3
     e.sp1 = sp;
     switch (pr.kind) {
     case PT_PROC_ELASTIC: {  // src/relativistic_coulomb.jl:10-23
-        e.phi = TWO_PI * g.next();
+        e.phi = phi >= 0.0 ? phi : TWO_PI * g.next();
         V3 v = velocity(sp, p);
         double beta = norm3(v) / C;
         double a = pr.c[0];
@@ -761,7 +763,7 @@ __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3
             E2 = pr.kind == PT_PROC_MOLLER ? moller_sample(pr.tcut, eng, g) : bhabha_sample(pr.tcut, eng, g);
             E1 = E0 - E2;
         }
-        e.phi = TWO_PI * g.next();
+        e.phi = phi >= 0.0 ? phi : TWO_PI * g.next();
         // Lehtinen 1999 angles
         e.nA = sqrt(E1 * E1 + 2.0 * MC2 * E1) / C;
         e.nB = sqrt(E2 * E2 + 2.0 * MC2 * E2) / C;
