@@ -416,7 +416,7 @@ typedef struct {
 
 /* fill a config with the defaults for `which` */
 int pt_learn_defaults(int which, pt_learn_config* cfg);
-/* generate the training set, train, build the BF16 image and judge it by the KS gate:
+/* generate the training set, train, build the BF16 image and check it against the KS gate:
  * enabled if it passes, otherwise the rejection sampler stays in use (fallback) */
 int pt_learn_train(pt_ctx* ctx, int which, const pt_learn_config* cfg, pt_learn_report* rep);
 /* use (on = 1) or stop using (0) a trained network */

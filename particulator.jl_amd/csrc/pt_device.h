@@ -360,7 +360,7 @@ __device__ __forceinline__ RbebSetup rbeb_setup(double T, double B) {
 // One trip: w (exact) and whether it is accepted.  The reference's 1/(1+w)^k and
 // 1/(t-w)^k come from 1/(1+w) = (ratio - u) / ratio and one reciprocal (two divisions
 // instead of eight): the acceptance terms are within a few ulps of the reference's, so
-// only a z * pb within ~1e-15 of p0 could be judged otherwise.
+// only a z * pb within ~1e-15 of p0 could be decided otherwise.
 __device__ __forceinline__ bool rbeb_trip(const RbebSetup& s, double u, double z, double& w) {
     const double den = s.ratio - u;
     w = u / den;

@@ -27,7 +27,7 @@ def defaults(which):
 
 
 def train(mp, which, **overrides):
-    """Train the sampler on the device from the context's own samplers and judge it by the
+    """Train the sampler on the device from the context's own samplers and check it against the
     held-out KS gate; it is used by the step if it passes.  Returns the report as a dict."""
     d = defaults(which)
     d.update(overrides)
