@@ -231,44 +231,69 @@ FETCH_SCALE = 2.0  # FETCH_SIZE reads 0.5 GiB for a 1 GiB device copy on this GP
 WRITE_SCALE = 1.0  # WRITE_SIZE reads 1 GiB for the same copy
 
 
-def pmc_traffic(args):
-    """HBM bytes per launch of the step's advance kernel from two rocprofv3 counter passes
-    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of a short run of this workload, scaled
-    by what the counters read for a plain device copy of known size."""
+VALU_COUNTERS = ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVE_CYCLES", "SQ_WAIT_ANY")
+
+
+def pmc_pass(args, counters):
+    """Per-dispatch values of `counters` for the main passes of the step's advance kernel,
+    from one rocprofv3 counter pass over a short run of this workload (a child process)."""
     import shutil
     import sqlite3
     import subprocess
     import tempfile
     exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
-    kib = {}
-    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-        d = tempfile.mkdtemp(prefix="pt_pmc_", dir="/tmp")
-        cmd = [exe, "--pmc", ctr, "-d", d, "-o", "p", "--", sys.executable, os.path.abspath(__file__),
-               "--workload", args.workload, "--steps", "3", "--warmup", "1", "--no-learned",
-               "--no-cpu-baseline", "--no-traffic"] + (["--n", str(args.n)] if args.n else [])
-        log("counter pass %s" % ctr)
-        subprocess.run(cmd, timeout=240, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd="/tmp",
-                       env=dict(os.environ, TMPDIR="/tmp"), check=False)
-        dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("_results.db")]
-        if not dbs:
-            raise RuntimeError("no %s results" % ctr)
-        db = sqlite3.connect(dbs[0])
+    d = tempfile.mkdtemp(prefix="pt_pmc_", dir="/tmp")
+    cmd = [exe, "--pmc"] + list(counters) + ["-d", d, "-o", "p", "--", sys.executable, os.path.abspath(__file__),
+                                             "--workload", args.workload, "--steps", "3", "--warmup", "1",
+                                             "--no-learned", "--no-cpu-baseline", "--no-traffic"]
+    cmd += ["--n", str(args.n)] if args.n else []
+    log("counter pass %s" % " ".join(counters))
+    subprocess.run(cmd, timeout=240, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd="/tmp",
+                   env=dict(os.environ, TMPDIR="/tmp"), check=False)
+    dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("_results.db")]
+    if not dbs:
+        raise RuntimeError("no results for %s" % (counters,))
+    db = sqlite3.connect(dbs[0])
+    out = {}
+    for ctr in counters:
         rows = db.execute("select value from counters_collection where counter_name = ? and "
                           "kernel_name like '%k_advance<%, 0, false, false>%'", (ctr,)).fetchall()
-        db.close()
-        shutil.rmtree(d, ignore_errors=True)
         vals = [r[0] for r in rows]
         if not vals:
             raise RuntimeError("no %s samples for the advance kernel" % ctr)
         # the step's main pass (the short passes over overflow secondaries are left out)
         main = [v for v in vals if v >= 0.1 * max(vals)]
-        kib[ctr] = (sum(main) / len(main), len(main))
+        out[ctr] = (sum(main) / len(main), len(main))
+    db.close()
+    shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
+def pmc_traffic(args):
+    """HBM bytes per launch of the step's advance kernel from two counter passes
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), scaled by what the counters read for a
+    plain device copy of known size; and, from a third pass, its vector-instruction figures."""
+    kib = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        kib.update(pmc_pass(args, (ctr,)))
     fetch = kib["FETCH_SIZE"][0] * 1024 * FETCH_SCALE
     write = kib["WRITE_SIZE"][0] * 1024 * WRITE_SCALE
-    return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
-            "fetch_kib_raw": kib["FETCH_SIZE"][0], "write_kib_raw": kib["WRITE_SIZE"][0],
-            "dispatches": kib["FETCH_SIZE"][1], "scale": {"FETCH_SIZE": FETCH_SCALE, "WRITE_SIZE": WRITE_SCALE},
-            "note": "counter passes of a 3-step run of the same workload"}
+    out = {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
+           "fetch_kib_raw": kib["FETCH_SIZE"][0], "write_kib_raw": kib["WRITE_SIZE"][0],
+           "dispatches": kib["FETCH_SIZE"][1], "scale": {"FETCH_SIZE": FETCH_SCALE, "WRITE_SIZE": WRITE_SCALE},
+           "note": "counter passes of a 3-step run of the same workload"}
+    try:
+        v = {k: x[0] for k, x in pmc_pass(args, VALU_COUNTERS).items()}
+        n0 = args.n or WORKLOADS[args.workload]["n"]
+        out["valu"] = {"wave_instructions_per_launch": v["SQ_INSTS_VALU"],
+                       "wave_instructions_per_particle_step": v["SQ_INSTS_VALU"] / n0,
+                       "lane_utilisation": v["SQ_THREAD_CYCLES_VALU"] / (64.0 * v["SQ_ACTIVE_INST_VALU"]),
+                       "wait_fraction": v["SQ_WAIT_ANY"] / v["SQ_WAVE_CYCLES"],
+                       "note": "the main passes of the same 3-step run: SQ_THREAD_CYCLES_VALU / "
+                               "(64 SQ_ACTIVE_INST_VALU), SQ_WAIT_ANY / SQ_WAVE_CYCLES"}
+    except Exception as e:  # the byte figures stand without it
+        out["valu"] = {"error": repr(e)}
+    return out
 
 
 def run_beam(args):
