@@ -1,0 +1,54 @@
+"""The runaway-electron swarm at its full size and length (the reference's CPU-runnable
+case, scripts/swarm.jl): 10^4 electrons at 1 MeV, isotropic, in dry air at E = 1 MV/m
+along -z, 10 keV cut, dt = 1 ps, 2000 steps, photons tallied at birth, 32 independent
+samples.  The device runs the 32 samples as 32 swarms of one population; the oracle's 32
+samples (other seeds) are the committed fixture tests/golden/c1_oracle.json, written by
+tests/golden/make_c1_oracle.py.  Growth rate and mean energy agree within 2 sigma."""
+import json
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+import particulator_hip as P
+from particulator_hip import air, co
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_c1_swarm_growth_and_mean_energy_32_samples():
+    ref = json.load(open(os.path.join(HERE, "golden", "c1_oracle.json")))
+    NS, N, STEPS, DT, E = len(ref["count"]), ref["n0"], ref["steps"], ref["dt"], 1e6
+    te, tg, tp = air.swarm_tables(air.composition(), [E] * NS, DT, H.CUT)
+    mp = P.MultiPopulation(seed=71, nswarm=NS, tally_bins=64, tally_range=(1e4 * co.eV, 1e7 * co.eV),
+                           electron=P.Population(1 << 21, [], te, H.CUT, tracked=True),
+                           photon=P.Population(1024, [], tg, H.CUT, tracked=False),
+                           positron=P.Population(1 << 16, [], tp, H.CUT, tracked=False))
+    for i in range(NS):
+        mp.set_pusher(P.RK2Pusher(P.ElectromagneticField(P.HomogeneousField((0.0, 0.0, -E)))), swarm=i)
+        mp.init_particles(P.PT_ELECTRON, N, 1e6 * co.eV, swarm=i, uid0=i * N)
+    mp.init()
+    say("device: %d swarms x %d e-, %d steps" % (NS, N, STEPS))
+    mp.run_steps(0.0, DT, STEPS, compact_every=8, check_every=200)
+    st = mp.swarm_stats(P.PT_ELECTRON)
+    mp.close()
+    T = STEPS * DT
+    dev_g = np.log(st[:, 0] / N) / T
+    dev_e = st[:, 2] / st[:, 1] / co.eV
+    cpu_g = np.log(np.array(ref["count"], dtype=float) / N) / T
+    cpu_e = np.array(ref["mean_energy_eV"])
+    for name, x, y in (("growth rate (1/s)", dev_g, cpu_g), ("mean energy (eV)", dev_e, cpu_e)):
+        sig = math.sqrt(x.var(ddof=1) / x.size + y.var(ddof=1) / y.size)
+        say("%s: device %.5g +- %.2g, oracle %.5g +- %.2g, |diff| / sigma = %.2f"
+            % (name, x.mean(), x.std(ddof=1) / math.sqrt(x.size), y.mean(), y.std(ddof=1) / math.sqrt(y.size),
+               abs(x.mean() - y.mean()) / sig))
+        assert abs(x.mean() - y.mean()) <= 2 * sig
