@@ -1,0 +1,63 @@
+"""Edge cases of the step on the device: empty populations, a population that reaches its
+capacity (add_particle!'s error, src/population.jl:107) and an energy above the table's
+range (the reference would read past its coefficients, src/cheby.jl:127-143): the step ends
+normally and the condition comes back as a status code."""
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle as O
+from particulator_hip import PTError, co
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_empty_populations_step():
+    te, tg, tp = H.tables()
+    mp = H.make_device(te, tg, tp, 3)
+    mp.init()
+    say("empty populations, 5 steps")
+    mp.run_steps(0.0, 1e-12, 5, compact_every=2, check_every=1)
+    assert [mp.count(s) for s in range(3)] == [0, 0, 0]
+    assert mp.download(0).size == 0
+    mp.close()
+
+
+def test_capacity_overflow_is_reported():
+    te, tg, tp = H.tables()
+    n = 1000
+    mp = H.make_device(te, tg, tp, 4, caps=(n + 8, 1 << 12, 1 << 12))
+    mp.init_particles(0, n, 5e6 * co.eV)
+    mp.init()
+    say("capacity %d for %d electrons at 5 MeV, up to 400 steps" % (n + 8, n))
+    with pytest.raises(PTError) as e:
+        mp.run_steps(0.0, 1e-12, 400, compact_every=0, check_every=50)
+    assert e.value.code == -3
+    arr = mp.download(0)
+    assert arr.size <= n + 8
+    assert np.unique(arr["uid"]).size == arr.size
+    mp.close()
+
+
+def test_energy_above_table_range_is_reported():
+    te, tg, tp = H.tables()
+    mp = H.make_device(te, tg, tp, 5)
+    init = O.init_swarm(64, 0, 5, e1=1e6)
+    init["p"] *= 0.0
+    pz = np.sqrt((1e9 * co.eV + co.electron_mc2) ** 2 - co.electron_mc2 ** 2) / co.c  # 1 GeV
+    init["p"][:, 2] = pz
+    soa, uid, sw = H.soa_of(init)
+    mp.add_soa(0, soa, uid=uid, swarm=sw)
+    say("64 electrons at 1 GeV (table up to %.3g eV)" % (te.b.xmax / co.eV))
+    with pytest.raises(PTError) as e:
+        mp.init()
+        mp.run_steps(0.0, 1e-12, 1, compact_every=0, check_every=1)
+    assert e.value.code == -7
+    mp.close()
