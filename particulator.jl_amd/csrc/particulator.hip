@@ -742,7 +742,10 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, char* w
                 if (e.dep != 0.0) deposit(P, sh.bc, e.dep * w, x.z);
                 V3 m0, m1;
                 double s0, s1;
-                finish_event(p, pn, e, g, m0, m1, s0, s1, sh.cut, sh.stored);
+                bool drop1;
+                // without a dose tally a secondary below the cut leaves nothing behind
+                const double dropcut = P.dose_bins > 0 ? -1.0 : sh.cut[e.sp1] * (1.0 - 1e-9);
+                finish_event(p, pn, e, g, m0, m1, s0, s1, sh.cut, sh.stored, dropcut, drop1);
                 uint32_t dend = g.draw;
                 if (e.keep) {
                     p = m0;
@@ -762,7 +765,7 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, char* w
                         nch++;
                     }
                 }
-                if (e.ne == 2) {
+                if (e.ne == 2 && !drop1) {
                     ull sl = append(P, sh, e.sp1, x, m1, w, t, s1, uid, step, dend, e.keep ? 0 : 1, swarm, main_pass,
                                     tfinal);
                     if (sl != NO_SLOT && nch < NCHILD) {
@@ -1045,7 +1048,8 @@ __global__ void __launch_bounds__(BS) k_sample(const Params* __restrict__ Pp) {
             if (e.dep != 0.0) deposit(P, sh.bc, e.dep * w, x.z);
             V3 m0, m1;
             double s0, s1;
-            finish_event(p, pn, e, g, m0, m1, s0, s1, sh.cut, sh.stored);
+            bool drop1;
+            finish_event(p, pn, e, g, m0, m1, s0, s1, sh.cut, sh.stored, -1.0, drop1);
             const uint32_t dend = g.draw;
             if (e.keep) {
                 bool oob = false;
@@ -1909,7 +1913,8 @@ __global__ void __launch_bounds__(256) k_dbg_collide(const Params* __restrict__ 
     sample_event(P.tab[sp].procs[proc], sp, p, pn, E, P.sb, P.pe, g, e);
     V3 m0, m1;
     double s0, s1;
-    finish_event(p, pn, e, g, m0, m1, s0, s1, nullptr, nullptr);
+    bool drop1;
+    finish_event(p, pn, e, g, m0, m1, s0, s1, nullptr, nullptr, -1.0, drop1);
     int kind = e.keep ? (e.ne == 1 ? 1 : 2) : (e.ne == 1 ? 3 : 4);
     okind[i] = g.ran_out ? -2 : (e.err ? -1 - 16 * e.err : kind);
     odraws[i] = (int32_t)g.draw;

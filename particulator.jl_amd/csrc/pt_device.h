@@ -715,6 +715,7 @@ enum { SRC_A = 0, SRC_B = 1, SRC_PMA = 2 };
 struct Ev {
     double cA, nA, cB, nB, phi;
     double dep;  // energy left at the collision point (binding energy)
+    double eB;   // kinetic energy of the particle taking turn B when it is known, else -1
     int nturn, sgnB, keep, ne;
     int src0, src1, sp0, sp1;
     int err;
@@ -732,6 +733,7 @@ __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3
     e.cB = 0.0;
     e.nB = 0.0;
     e.dep = 0.0;
+    e.eB = -1.0;
     e.src1 = SRC_A;
     e.sp1 = sp;
     switch (pr.kind) {
@@ -767,6 +769,7 @@ __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3
         // Lehtinen 1999 angles
         e.nA = sqrt(E1 * E1 + 2.0 * MC2 * E1) / C;
         e.nB = sqrt(E2 * E2 + 2.0 * MC2 * E2) / C;
+        e.eB = E2;
         e.cA = sqrt(E1 * (E0 + 2.0 * MC2) / (E0 * (E1 + 2.0 * MC2)));
         e.cB = sqrt(E2 * (E0 + 2.0 * MC2) / (E0 * (E2 + 2.0 * MC2)));
         e.nturn = 2;
@@ -882,6 +885,7 @@ template <class G>
 __device__ __forceinline__ void learned_event(const DevProc& pr, int sp, double eng, float y0, float y1, G& g, Ev& e) {
     e.err = 0;
     e.dep = 0.0;
+    e.eB = -1.0;
     e.nturn = 1;
     e.sgnB = 1;
     e.cB = 0.0;
@@ -912,6 +916,7 @@ __device__ __forceinline__ void learned_event(const DevProc& pr, int sp, double 
         e.phi = TWO_PI * g.next();
         e.nA = sqrt(E1 * E1 + 2.0 * MC2 * E1) / C;
         e.nB = sqrt(E2 * E2 + 2.0 * MC2 * E2) / C;
+        e.eB = E2;
         e.cA = sqrt(E1 * (E0 + 2.0 * MC2) / (E0 * (E1 + 2.0 * MC2)));
         e.cB = sqrt(E2 * (E0 + 2.0 * MC2) / (E0 * (E2 + 2.0 * MC2)));
         e.nturn = 2;
@@ -930,10 +935,14 @@ __device__ __forceinline__ void learned_event(const DevProc& pr, int sp, double 
 // cut[sp] / tracked[sp] (NULL: every s is computed) tell which new particles add_particle!
 // drops (at or below the cut, or not tracked): their s is never read, so its draw only
 // advances the stream.
+// dropcut > 0: entry 1 taking turn B with a known energy below dropcut is dropped by the
+// caller without looking at it (no deposit is kept): its turn is not computed, m1 = 0 and
+// drop1 is set; dropcut sits far enough below the cut that the exact test would drop it too.
 template <class G>
 __device__ __forceinline__ void finish_event(const V3& p, double pnorm, const Ev& e, G& g, V3& m0,
                                              V3& m1, double& s0, double& s1, const double* cut,
-                                             const int* tracked) {
+                                             const int* tracked, double dropcut, bool& drop1) {
+    drop1 = e.ne == 2 && e.src1 == SRC_B && e.eB >= 0.0 && e.eB < dropcut;
     double u0 = p.x / pnorm, u1 = p.y / pnorm, u2 = p.z / pnorm;
     double sp, cp;
     sincos(e.phi, &sp, &cp);
@@ -957,7 +966,7 @@ __device__ __forceinline__ void finish_event(const V3& p, double pnorm, const Ev
         }
         pa = V3{e.nA * x, e.nA * y, e.nA * z};
     }
-    if (e.nturn == 2) {
+    if (e.nturn == 2 && !drop1) {
         double spb = e.sgnB > 0 ? sp : -sp;
         double q = 1.0 - e.cB * e.cB;
         double st = sqrt(q > 0.0 ? q : 0.0);
@@ -991,7 +1000,7 @@ __device__ __forceinline__ void finish_event(const V3& p, double pnorm, const Ev
     }
     s1 = 0.0;
     if (e.ne == 2) {
-        if (stored(e.sp1, m1))
+        if (!drop1 && stored(e.sp1, m1))
             s1 = nextcoll(g);
         else
             g.skip();
