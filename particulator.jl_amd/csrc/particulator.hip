@@ -736,15 +736,15 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, char* w
                 const int kind = sh.procs[sp][csel].kind;
                 double phi = -1.0;
                 if (kind == PT_PROC_ELASTIC || (kind == PT_PROC_RBEB && rb_w >= 0.0)) phi = TWO_PI * g.next();
+                // without a dose tally a secondary below the cut leaves nothing behind
+                const double dropcut = P.dose_bins > 0 ? -1.0 : sh.cut[PT_ELECTRON] * (1.0 - 1e-9);
                 Ev e;
-                sample_event(sh.procs[sp][csel], sp, p, pn, E, P.sb, P.pe, g, e, rb_w, phi);
+                sample_event(sh.procs[sp][csel], sp, p, pn, E, P.sb, P.pe, g, e, rb_w, phi, dropcut);
                 if (e.err) atomicAdd(&sh.bc.flags[FLAG_SAMPLER], 1ull);
                 if (e.dep != 0.0) deposit(P, sh.bc, e.dep * w, x.z);
                 V3 m0, m1;
                 double s0, s1;
                 bool drop1;
-                // without a dose tally a secondary below the cut leaves nothing behind
-                const double dropcut = P.dose_bins > 0 ? -1.0 : sh.cut[e.sp1] * (1.0 - 1e-9);
                 finish_event(p, pn, e, g, m0, m1, s0, s1, sh.cut, sh.stored, dropcut, drop1);
                 uint32_t dend = g.draw;
                 if (e.keep) {

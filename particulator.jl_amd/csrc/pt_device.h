@@ -724,7 +724,8 @@ struct Ev {
 template <class G>
 __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3& p, double pnorm,
                                              double eng, const DevSB* sbs, const DevPE* pes, G& g,
-                                             Ev& e, double rbeb_w = -1.0, double phi = -1.0) {
+                                             Ev& e, double rbeb_w = -1.0, double phi = -1.0,
+                                             double dropcut = -1.0) {
     // phi >= 0: the azimuth was drawn by the caller at the point where this event draws
     // it (elastic: first; RBEB after a finished rejection loop: first)
     e.err = 0;
@@ -766,12 +767,15 @@ __device__ __forceinline__ void sample_event(const DevProc& pr, int sp, const V3
             E1 = E0 - E2;
         }
         e.phi = phi >= 0.0 ? phi : TWO_PI * g.next();
-        // Lehtinen 1999 angles
+        // Lehtinen 1999 angles; those of a secondary the caller drops unseen (E2 < dropcut,
+        // see finish_event) are not needed
         e.nA = sqrt(E1 * E1 + 2.0 * MC2 * E1) / C;
-        e.nB = sqrt(E2 * E2 + 2.0 * MC2 * E2) / C;
         e.eB = E2;
         e.cA = sqrt(E1 * (E0 + 2.0 * MC2) / (E0 * (E1 + 2.0 * MC2)));
-        e.cB = sqrt(E2 * (E0 + 2.0 * MC2) / (E0 * (E2 + 2.0 * MC2)));
+        if (!(E2 < dropcut)) {
+            e.nB = sqrt(E2 * E2 + 2.0 * MC2 * E2) / C;
+            e.cB = sqrt(E2 * (E0 + 2.0 * MC2) / (E0 * (E2 + 2.0 * MC2)));
+        }
         e.nturn = 2;
         e.sgnB = -1;
         e.keep = 1;
