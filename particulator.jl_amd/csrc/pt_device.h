@@ -141,6 +141,17 @@ struct PhiloxStream {
         if (cblk != blk + 1) fill(blk);
         return (d & 1) ? c1 : c0;
     }
+    // the next k = 1 or 2 uniforms (b is set when k == 2), as k next() calls return them,
+    // with at most one Philox block computed, at one place
+    __device__ __forceinline__ void next12(int k, double& a, double& b) {
+        const uint32_t d = draw;
+        draw = d + (uint32_t)k;
+        const double first = c1;  // draw d when d is odd (the invariant)
+        const uint32_t need = (d + (uint32_t)k - 1u) >> 1;
+        if (cblk != need + 1) fill(need);
+        a = (d & 1) ? first : c0;
+        b = (d & 1) ? c0 : c1;
+    }
     // the next two uniforms, as two next() calls return them, with at most one Philox block
     // computed at one place: in a rejection loop every lane needs its block at the same
     // instruction, instead of half of the lanes at each of two draw sites
@@ -174,6 +185,10 @@ struct InjectedStream {
     __device__ __forceinline__ void next2(double& a, double& b) {
         a = next();
         b = next();
+    }
+    __device__ __forceinline__ void next12(int k, double& a, double& b) {
+        a = next();
+        b = k == 2 ? next() : 0.0;
     }
 };
 
@@ -996,19 +1011,12 @@ __device__ __forceinline__ void finish_event(const V3& p, double pnorm, const Ev
     auto stored = [&](int spk, const V3& m) {
         return !cut || (tracked[spk] && kinenergy(spk, m) > cut[spk]);
     };
-    if (e.keep || stored(e.sp0, m0)) {
-        s0 = nextcoll(g);
-    } else {
-        g.skip();
-        s0 = 0.0;
-    }
-    s1 = 0.0;
-    if (e.ne == 2) {
-        if (!drop1 && stored(e.sp1, m1))
-            s1 = nextcoll(g);
-        else
-            g.skip();
-    }
+    const bool need0 = e.keep || stored(e.sp0, m0);
+    const bool need1 = e.ne == 2 && !drop1 && stored(e.sp1, m1);
+    double q0, q1;
+    g.next12(e.ne == 2 ? 2 : 1, q0, q1);  // both draws at one place
+    s0 = need0 ? -log(q0) : 0.0;          // nextcoll(), src/util.jl:17
+    s1 = need1 ? -log(q1) : 0.0;
 }
 
 }  // namespace pt
