@@ -33,6 +33,7 @@
 namespace pt {
 // the message of the last failed context-free call on this host thread (pt_tables.cpp)
 const char* host_last_error();
+void host_set_error(const char* msg);
 }  // namespace pt
 
 using namespace pt;
@@ -2148,14 +2149,17 @@ int pt_create(const pt_config* cfg, pt_ctx** out) {
     c->cfg = *cfg;
     if (c->cfg.nswarm < 1) c->cfg.nswarm = 1;
     int ndev = 0;
+    // failures that return no context leave their message for pt_last_error(NULL)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
         int rc = fail(c, PT_E_NOT_BUILT, "no HIP device available");
-        pt_destroy(c);
+        host_set_error(c->err.c_str());
+        delete c;
         return rc;
     }
     for (int s = 0; s < PT_NSPECIES; s++) {
         if (c->cfg.capacity[s] >= (1ull << 31)) {
             int rc = fail(c, PT_E_ARG, "capacity of species %d exceeds 2^31 - 1", s);
+            host_set_error(c->err.c_str());
             pt_destroy(c);
             return rc;
         }

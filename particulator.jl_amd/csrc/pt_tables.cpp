@@ -371,6 +371,7 @@ double cheb_slope(const double* a, int n, double xi, double lo, double hi) {
 }  // namespace
 
 const char* host_last_error() { return g_err.empty() ? "null context" : g_err.c_str(); }
+void host_set_error(const char* msg) { g_err = msg; }
 
 }  // namespace pt
 
