@@ -1,0 +1,111 @@
+"""The coupled avalanche at its full size (10^7 electrons log-uniform over 0.1-10 MeV, dry
+air, E = 1 MV/m, dt = 1 ps, electrons, photons and positrons tracked, capacity 2^27,
+compaction every 8 steps), checked through properties that hold at any size:
+
+- the populations do not depend on the launch shape, the secondary passes, collision
+  rounds or graph replay: a host-stepped run, a graph-replayed run with another launch and
+  a run with a collision round are equal bit for bit;
+- every uid is unique, every active particle ends at the step's time, none is below the
+  energy cut (droplow!, src/population.jl:273-284), and none is lost by compaction;
+- the CollisionCounter totals (src/callback.jl:118-141) equal the selections the step made,
+  and the particle-steps counted equal the active particles at the start of each step.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import particulator_hip as P
+from particulator_hip import air, co
+
+pytestmark = pytest.mark.gpu
+
+N = 10_000_000
+CAP = 1 << 27
+STEPS = 24
+DT = 1e-12
+CUT = 1e4 * co.eV
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def tables():
+    return air.build_tables(air.composition(), 1e6, DT, CUT)
+
+
+def world(tables, seed):
+    te, tg, tp = tables
+    mp = P.MultiPopulation(seed=seed, electron=P.Population(CAP, [], te, CUT),
+                           photon=P.Population(CAP, [], tg, CUT), positron=P.Population(CAP, [], tp, CUT))
+    mp.set_pusher(P.RK2Pusher(P.ElectromagneticField(P.HomogeneousField((0.0, 0.0, -1e6)))))
+    mp.init_particles(P.PT_ELECTRON, N, 1e5 * co.eV, 1e7 * co.eV, loguniform=True)
+    mp.init()
+    return mp
+
+
+def kinetic(sp, p):
+    pc = np.linalg.norm(p, axis=1) * co.c
+    if sp == P.PT_PHOTON:
+        return pc
+    return np.sqrt(pc * pc + co.electron_mc2 ** 2) - co.electron_mc2
+
+
+def snapshot(mp):
+    out = {}
+    for sp in (P.PT_ELECTRON, P.PT_PHOTON, P.PT_POSITRON):
+        a = mp.download(sp)
+        out[sp] = a[np.argsort(a["uid"], kind="stable")]
+    return out
+
+
+def test_avalanche_full_size(tables):
+    seed = 4242
+    say("full-size avalanche, host-stepped: %d e-, %d steps" % (N, STEPS))
+    mp = world(tables, seed)
+    nact0 = sum(mp.stats(sp).nactive for sp in range(3))
+    mp.work(reset=True)
+    for sp in range(3):
+        mp.collision_counts(sp, reset=True)
+    t = mp.run_steps(0.0, DT, STEPS, compact_every=8, check_every=STEPS)
+    w = mp.work()
+    counts = sum(int(mp.collision_counts(sp).sum()) for sp in range(3))
+    nact = [mp.stats(sp).nactive for sp in range(3)]
+    nslots = [mp.count(sp) for sp in range(3)]
+    a = snapshot(mp)
+    mp.close()
+    say("actives after %d steps: %s, selections %d, particle-steps %d" % (STEPS, nact, counts, w.particle_steps))
+    assert nact[P.PT_ELECTRON] > 0.99 * N and nact[P.PT_PHOTON] > 0 and w.secondaries > 0
+    # the last step compacted: every slot below the count is active
+    assert nslots == nact
+    assert counts == w.events
+    # few particles fall below the cut in 24 ps, and few are born
+    assert 0.99 * nact0 * STEPS <= w.particle_steps <= 1.01 * nact0 * STEPS
+    for sp in range(3):
+        s = a[sp]
+        assert s.size == nact[sp]
+        assert np.unique(s["uid"]).size == s.size
+        assert np.all(np.abs(s["t"] - t) <= 1e-15)
+        assert np.all(kinetic(sp, s["p"]) >= CUT)
+        assert np.all(np.isfinite(s["x"])) and np.all(np.isfinite(s["p"]))
+
+    for label, launch, graph in (("graph replays with block 512, grid x2, 4 secondary passes",
+                                  dict(block=512, grid_mult=2, compact_every=8, tail_passes=4), True),
+                                 ("one collision round for bremsstrahlung and Moller",
+                                  dict(rounds=1, defer_mask=(1 << 2) | (1 << 4)), False)):
+        say("full-size avalanche, " + label)
+        mp = world(tables, seed)
+        mp.set_launch(**launch)
+        if graph:
+            mp.run_graph(0.0, DT, STEPS, check_every=STEPS)
+        else:
+            mp.run_steps(0.0, DT, STEPS, compact_every=8, check_every=STEPS)
+        b = snapshot(mp)
+        mp.close()
+        for sp in range(3):
+            assert a[sp].size == b[sp].size, (label, sp)
+            for f in ("uid", "x", "p", "w", "t", "s", "r", "swarm"):
+                assert np.array_equal(a[sp][f], b[sp][f]), (label, sp, f)
