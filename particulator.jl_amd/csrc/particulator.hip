@@ -948,7 +948,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) k_
 // apply!, src/collisions.jl:83-133), and queues the surviving primary and the new
 // particles for the next push/select round.
 template <int BS>
-__global__ void __launch_bounds__(BS) k_sample(const Params* __restrict__ Pp) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) k_sample(const Params* __restrict__ Pp) {
     extern __shared__ double lds[];
     __shared__ Shared sh;
     __shared__ ull seg[PT_NSPECIES * NKQ + 1];
