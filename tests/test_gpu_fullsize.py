@@ -109,3 +109,48 @@ def test_avalanche_full_size(tables):
             assert a[sp].size == b[sp].size, (label, sp)
             for f in ("uid", "x", "p", "w", "t", "s", "r", "swarm"):
                 assert np.array_equal(a[sp][f], b[sp][f]), (label, sp, f)
+
+
+def test_sweep_full_size_batching_changes_nothing():
+    """The batched sweep at its full size (256 swarms at 0.4-4 MV/m, U{10^3..10^6} electrons
+    each at 1 MeV, ~1.2x10^8 in all, photons tallied): a swarm advanced among the others
+    ends as it does advanced alone in its own population with the same field, rate bound and
+    uids.  Per-swarm counts, weights and maximum energies are compared exactly, the energy
+    sums to rounding (their summation order differs)."""
+    ns, steps, seed = 256, 6, 2024
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(1000, 1_000_001, size=ns)
+    fields = 0.4e6 * 10.0 ** (np.arange(ns) / (ns - 1))
+    te, tg, tp = air.swarm_tables(air.composition(), list(fields), DT, CUT)
+    cap = int(2 * counts.sum())
+    say("full-size sweep, batched: %d swarms, %d electrons, %d steps" % (ns, counts.sum(), steps))
+    mp = P.MultiPopulation(seed=seed, nswarm=ns, electron=P.Population(cap, [], te, CUT),
+                           photon=P.Population(1024, [], tg, CUT, tracked=False),
+                           positron=P.Population(1024, [], tp, CUT, tracked=False))
+    uid0 = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    for i in range(ns):
+        mp.set_pusher(P.RK2Pusher(P.ElectromagneticField(P.HomogeneousField((0.0, 0.0, -fields[i])))), swarm=i)
+        mp.init_particles(P.PT_ELECTRON, int(counts[i]), 1e6 * co.eV, swarm=i, uid0=int(uid0[i]))
+    mp.init()
+    mp.set_launch(block=512, grid_mult=2, compact_every=2)
+    mp.run_graph(0.0, DT, steps, check_every=steps)
+    batched = mp.swarm_stats(P.PT_ELECTRON)
+    mp.close()
+    assert batched[:, 0].sum() > 0.99 * counts.sum()
+    picks = sorted({0, ns - 1, int(np.argmin(counts)), int(np.argmax(counts)), ns // 2})
+    for i in picks:
+        say("swarm %d alone: %d electrons at %.3g V/m" % (i, counts[i], fields[i]))
+        one = P.MultiPopulation(seed=seed, electron=P.Population(2 * int(counts[i]) + 4096, [],
+                                                                 te.with_ratebounds(te.ratebound[i:i + 1]), CUT),
+                                photon=P.Population(1024, [], tg, CUT, tracked=False),
+                                positron=P.Population(1024, [], tp.with_ratebounds(tp.ratebound[i:i + 1]), CUT,
+                                                      tracked=False))
+        one.set_pusher(P.RK2Pusher(P.ElectromagneticField(P.HomogeneousField((0.0, 0.0, -fields[i])))))
+        one.init_particles(P.PT_ELECTRON, int(counts[i]), 1e6 * co.eV, uid0=int(uid0[i]))
+        one.init()
+        one.run_steps(0.0, DT, steps, compact_every=8, check_every=steps)
+        alone = one.swarm_stats(P.PT_ELECTRON)[0]
+        one.close()
+        b = batched[i]
+        assert b[0] == alone[0] and b[1] == alone[1] and b[3] == alone[3], (i, b, alone)
+        assert abs(b[2] - alone[2]) <= 1e-10 * abs(alone[2]), (i, b, alone)
