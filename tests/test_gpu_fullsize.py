@@ -154,3 +154,47 @@ def test_sweep_full_size_batching_changes_nothing():
         b = batched[i]
         assert b[0] == alone[0] and b[1] == alone[1] and b[3] == alone[3], (i, b, alone)
         assert abs(b[2] - alone[2]) <= 1e-10 * abs(alone[2]), (i, b, alone)
+
+
+def test_beam_full_size_launch_independent():
+    """The photon pencil beam at its full size (10^8 photons at 10 MeV along +z, no field,
+    1 ns steps, every species tracked down to 10 keV, depth-dose tally over 0-3 m and the
+    photon-energy birth tally): a host-stepped run and a graph-replayed run with another
+    launch and fewer secondary passes give the same populations (active counts, weights and
+    maximum energies exactly) and the same tallies (birth counts exactly, deposited energy
+    to rounding: FP64 atomics add in a different order)."""
+    n, steps, seed = 100_000_000, 4, 2024
+    te, tg, tp = air.build_tables(air.composition(), 1e6, 1e-12, CUT)
+    out = []
+    for label, launch, graph in (("host-stepped, 6 secondary passes", dict(tail_passes=6), False),
+                                 ("graph replays, block 512, grid x2, 3 secondary passes",
+                                  dict(block=512, grid_mult=2, compact_every=8, tail_passes=3), True)):
+        say("full-size beam, %s: %d photons, %d steps" % (label, n, steps))
+        mp = P.MultiPopulation(seed=seed, tally_bins=512, tally_range=(1e4 * co.eV, 1e7 * co.eV),
+                               dose=(300, 0.0, 3.0), tally_tracked=True,
+                               electron=P.Population(1 << 23, [], te, CUT),
+                               photon=P.Population(int(1.1 * n), [], tg, CUT),
+                               positron=P.Population(1 << 22, [], tp, CUT))
+        mp.set_pusher(P.RK2Pusher(P.ElectromagneticField(P.HomogeneousField((0.0, 0.0, 0.0)))))
+        mp.init_beam(P.PT_PHOTON, n, 1e7 * co.eV)
+        mp.init()
+        mp.set_launch(**dict(mp.get_launch(), **launch))
+        if graph:
+            mp.run_graph(0.0, 1e-9, steps, check_every=steps)
+        else:
+            mp.run_steps(0.0, 1e-9, steps, compact_every=8, check_every=steps)
+        st = [mp.stats(sp) for sp in range(3)]
+        r = {"n": [s.nactive for s in st], "w": [s.weight for s in st], "max": [s.max_energy for s in st],
+             "dose": mp.dose(), "births": [mp.tally(sp)[1] for sp in range(3)]}
+        mp.close()
+        say("  actives %s, births %s, dose %.6g J" % (r["n"], r["births"], r["dose"][3]))
+        out.append(r)
+    a, b = out
+    assert a["n"][P.PT_ELECTRON] > 0 and a["n"][P.PT_PHOTON] > 0.9 * n
+    assert a["n"] == b["n"] and a["w"] == b["w"] and a["max"] == b["max"] and a["births"] == b["births"]
+    ha, ua, oa, ta = a["dose"]
+    hb, ub, ob, tb = b["dose"]
+    assert ta > 0 and abs(ta - tb) <= 1e-10 * ta
+    assert np.allclose(ha, hb, rtol=1e-9, atol=1e-12 * ta)
+    # deposited energy is below the beam's energy
+    assert ta < n * 1e7 * co.eV
