@@ -1,7 +1,8 @@
 """The C host (tests/c_host/swarm.c) runs a runaway-electron swarm in dry air with tables it
 built through the ABI's host entry points; the Python host, with tables from the same
-builders (tables.native_table, SeltzerBerger(native=True)), gives the same swarm bit for
-bit (the random streams are keyed by seed, uid and step only)."""
+builders (tables.native_table, SeltzerBerger(native=True)), gives the same swarm (the random
+streams are keyed by seed, uid and step only): the same count, weight and maximum energy,
+and the mean energy to rounding."""
 import json
 import subprocess
 import sys
@@ -48,4 +49,8 @@ def test_c_host_matches_python_host(tmp_path):
     p = python_host(n, steps, seed)
     print(c, p, flush=True)
     assert c["nactive"] > n  # the swarm grows at 1 MV/m
-    assert c == p
+    # the particles are the same; the mean is a sum over slots, whose order follows the
+    # atomic slot claims, so it agrees to rounding
+    for k in ("nactive", "weight", "max_energy"):
+        assert c[k] == p[k], k
+    assert abs(c["mean_energy"] - p["mean_energy"]) <= 1e-12 * p["mean_energy"]
