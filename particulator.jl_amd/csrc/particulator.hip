@@ -140,20 +140,22 @@ struct Params {
 // Per-species pointers, table metadata and process records are staged in LDS once per
 // workgroup; the Chebyshev rate tables (and the rate bounds of a single swarm) follow in
 // dynamic LDS.
+template <typename T>
+using gptr = T PT_GLOBAL*;
 struct SpLite {
-    double* f[PT_NFIELDS];
-    uint8_t* active;
-    uint64_t* uid;
-    int32_t* swarm;
-    double* trem;
-    uint32_t* draw;
-    uint8_t* evp;
+    gptr<double> f[PT_NFIELDS];
+    gptr<uint8_t> active;
+    gptr<uint64_t> uid;
+    gptr<int32_t> swarm;
+    gptr<double> trem;
+    gptr<uint32_t> draw;
+    gptr<uint8_t> evp;
     ull* n;
     ull cap;
     const double* rate;  // LDS
     const double* rb;    // LDS (one swarm) or global
     double cut, xmax, ixmax;  // ixmax = 1 / xmax
-    int tracked, valid, nprocs, k, nswarm, pad;
+    int tracked, valid, nprocs, k, nswarm, rb_lds;
 };
 
 // block-local counters, flushed once per block (the depth-dose histogram too, when it has
@@ -182,13 +184,13 @@ __device__ __forceinline__ void stage(const Params& P, double* lds, Shared& sh) 
         const SpeciesDev& S = P.sp[s];
         if (threadIdx.x == 0) {
             SpLite& L = sh.sp[s];
-            for (int f = 0; f < PT_NFIELDS; f++) L.f[f] = S.f[f];
-            L.active = S.active;
-            L.uid = S.uid;
-            L.swarm = S.swarm;
-            L.trem = S.trem;
-            L.draw = S.draw;
-            L.evp = S.evp;
+            for (int f = 0; f < PT_NFIELDS; f++) L.f[f] = (gptr<double>)S.f[f];
+            L.active = (gptr<uint8_t>)S.active;
+            L.uid = (gptr<uint64_t>)S.uid;
+            L.swarm = (gptr<int32_t>)S.swarm;
+            L.trem = (gptr<double>)S.trem;
+            L.draw = (gptr<uint32_t>)S.draw;
+            L.evp = (gptr<uint8_t>)S.evp;
             L.n = S.n;
             L.cap = S.cap;
             L.tracked = S.tracked;
@@ -201,6 +203,7 @@ __device__ __forceinline__ void stage(const Params& P, double* lds, Shared& sh) 
             L.ixmax = 1.0 / tb.xmax;
             L.rate = lds + P.lds_rate_off[s];
             L.rb = P.rb_in_lds ? lds + P.lds_rb_off[s] : tb.ratebound;
+            L.rb_lds = P.rb_in_lds;
             sh.cut[s] = tb.cut;
             sh.stored[s] = S.tracked && tb.valid;
         }
@@ -225,8 +228,8 @@ __device__ __forceinline__ double setr(const SpLite& L, double E, int swarm, boo
     bool o;
     int i = precheb(E, L.k, L.xmax, L.ixmax, t1, t2, o);
     oob |= o;
-    const double* rb = L.rb + (size_t)(L.nswarm > 1 ? swarm : 0) * (L.k + 1) * 3 + i * 3;
-    return cheb3(rb, t1, t2);
+    const size_t off = (size_t)(L.nswarm > 1 ? swarm : 0) * (L.k + 1) * 3 + i * 3;
+    return L.rb_lds ? cheb3(as_lds(L.rb) + off, t1, t2) : cheb3(as_global(L.rb) + off, t1, t2);
 }
 
 __device__ void flush_counters(const Params& P, BlockCounters* bc) {
@@ -289,7 +292,7 @@ __device__ __forceinline__ ull append(const Params& P, Shared& sh, int sp2, cons
         atomicAdd(&sh.bc.flags[FLAG_CAPACITY], 1ull);
         return NO_SLOT;
     }
-    ull slot = atomicAdd(L.n, 1ull);
+    ull slot = global_add(L.n, 1ull);
     if (slot >= L.cap) {
         atomicAdd(&sh.bc.flags[FLAG_CAPACITY], 1ull);
         return NO_SLOT;
@@ -377,7 +380,7 @@ __device__ __forceinline__ void wave_push(ull* cnt, uint32_t* arr, uint32_t val,
         bool mine = cnt != nullptr && (ull)(uintptr_t)cnt == key;
         uint64_t m = __ballot(mine);
         ull base = 0;
-        if (lane == leader) base = atomicAdd((ull*)(uintptr_t)key, (ull)__popcll(m));
+        if (lane == leader) base = global_add((ull*)(uintptr_t)key, (ull)__popcll(m));
         base = __shfl(base, leader);
         if (mine) arr[base + (ull)__popcll(m & ltmask)] = val;
         pend &= ~m;
@@ -568,7 +571,7 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, char* w
             if (cbeg >= cend) {
                 int fl = __ffsll((unsigned long long)__ballot(1)) - 1;
                 ull b = 0;
-                if (lane == fl) b = atomicAdd(cursor, (ull)CHUNK);
+                if (lane == fl) b = global_add(cursor, (ull)CHUNK);
                 b = __shfl(b, fl);
                 if (b >= total) {
                     drained = true;
@@ -669,14 +672,13 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, char* w
                     coll = false;
                     s -= dt * r;
                 }
-                if constexpr (WALL) {
-                    const V3 xo = x, po = p;
-                    const double to = t;
-                    push(sp, P.fields[swarm], x, p, t, dt);
-                    wall_check(P, sp, xo, po, to, x, p, w, t, uid, swarm, active);
-                } else {
-                    push(sp, P.fields[swarm], x, p, t, dt);
-                }
+                const V3 xo = x, po = p;
+                const double to = t;
+                if (P.nswarm == 1)
+                    push(sp, as_global(P.fields)[0], x, p, t, dt);
+                else
+                    push(sp, as_global(P.fields)[swarm], x, p, t, dt);
+                if constexpr (WALL) wall_check(P, sp, xo, po, to, x, p, w, t, uid, swarm, active);
                 if (coll && r != 0.0 && (!WALL || active)) {
                     double E = kinenergy(sp, p);
                     if (E >= L.cut) {
@@ -686,7 +688,7 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, char* w
                         int iv = precheb(E, L.k, L.xmax, L.ixmax, t1, t2, o);
                         oob |= o;
                         double xi = g.next() * r;
-                        const double* rt = L.rate + (size_t)iv * L.nprocs * 3;
+                        const PT_LDS double* rt = as_lds(L.rate) + (size_t)iv * L.nprocs * 3;
                         int sel = -1;
                         for (int j = 0; j < L.nprocs; j++) {
                             double nu = cheb3(rt + 3 * j, t1, t2);
@@ -1856,7 +1858,7 @@ __global__ void __launch_bounds__(256) k_dbg_select(const Params* __restrict__ P
     bool o;
     int iv = precheb(E[i], L.k, L.xmax, L.ixmax, t1, t2, o);
     double xi = u[i] * r;
-    const double* rt = L.rate + (size_t)iv * L.nprocs * 3;
+    const PT_LDS double* rt = as_lds(L.rate) + (size_t)iv * L.nprocs * 3;
     int sel = -1;
     for (int j = 0; j < L.nprocs; j++) {
         double nu = cheb3(rt + 3 * j, t1, t2);

@@ -15,6 +15,24 @@
 
 namespace pt {
 
+// Explicit address spaces for pointers held in structs (the compiler cannot trace them to
+// their allocation): a generic pointer becomes a flat access, whose wait covers both the
+// LDS and the global counters.
+#define PT_LDS __attribute__((address_space(3)))
+#define PT_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const PT_LDS T* as_lds(const T* p) {
+    return (const PT_LDS T*)p;
+}
+template <typename T>
+__device__ __forceinline__ const PT_GLOBAL T* as_global(const T* p) {
+    return (const PT_GLOBAL T*)p;
+}
+// atomicAdd on a counter in global memory, issued as a global (not flat) atomic
+__device__ __forceinline__ unsigned long long global_add(unsigned long long* p, unsigned long long v) {
+    return __hip_atomic_fetch_add((unsigned long long PT_GLOBAL*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 constexpr int MAX_PROCS = 24;
 constexpr int MAX_INTERVALS = 64;
 constexpr int MAX_SHELLS = 8;
@@ -280,7 +298,8 @@ __device__ __forceinline__ int precheb(double x, int k, double xmax, double ixma
 }
 
 // sum(ntuple(k -> c[k]*t[k], Val(3))), src/collision_table.jl:318,332
-__device__ __forceinline__ double cheb3(const double* c, double t1, double t2) {
+template <typename Ptr>
+__device__ __forceinline__ double cheb3(Ptr c, double t1, double t2) {
     return (c[0] * 1.0 + c[1] * t1) + c[2] * t2;
 }
 
@@ -631,7 +650,8 @@ __device__ __forceinline__ double annih_sample(double eng, G& g) {
 }
 
 // ---------------------------------------------------------------- fields and push
-__device__ __forceinline__ V3 field_eval(const FieldFn& f, const V3& x) {
+template <class FF>
+__device__ __forceinline__ V3 field_eval(const FF& f, const V3& x) {
     switch (f.kind) {
     case PT_FIELD_HOMOGENEOUS:
         return V3{f.v[0], f.v[1], f.v[2]};
@@ -655,12 +675,12 @@ __device__ __forceinline__ V3 field_eval(const FieldFn& f, const V3& x) {
 // force(::ElectromagneticField, s), src/field.jl:62-68
 // chebeval with `order` terms (src/cheby.jl:55-81, :111-143): T_j = (2 xi) T_{j-1} - T_{j-2},
 // summed left to right; the interval index is clamped to [0, k]
-__device__ __forceinline__ double chebeval_n(const double* c, int k, int order, double xmax, double ixmax,
-                                             double x) {
+template <typename Ptr>
+__device__ __forceinline__ double chebeval_n(Ptr c, int k, int order, double xmax, double ixmax, double x) {
     double t1, t2;
     bool oob;
     int i = precheb(x, k, xmax, ixmax, t1, t2, oob);
-    const double* a = c + (size_t)i * order;
+    Ptr a = c + (size_t)i * order;
     double tm2 = 1.0, tm1 = t1;
     double acc = a[0] * 1.0;
     if (order > 1) acc = acc + a[1] * t1;
@@ -673,7 +693,8 @@ __device__ __forceinline__ double chebeval_n(const double* c, int k, int order, 
     return acc;
 }
 
-__device__ __forceinline__ V3 force(int sp, const Field& f, const V3& x, const V3& p) {
+template <class FT>
+__device__ __forceinline__ V3 force(int sp, const FT& f, const V3& x, const V3& p) {
     if (sp == PT_PHOTON) return V3{0.0, 0.0, 0.0};
     V3 e = field_eval(f.e, x), b = field_eval(f.b, x);
     V3 v = velocity(sp, p);
@@ -685,8 +706,8 @@ __device__ __forceinline__ V3 force(int sp, const Field& f, const V3& x, const V
     if (f.cl.on) {
         // CombinedForcing(field, ContinuumLoss): the sum of the forces (src/pusher.jl:14-23);
         // the loss term is p * (-f / |p|) (src/continuum.jl:17-20)
-        double fc = chebeval_n(f.cl.c[sp == PT_ELECTRON ? 0 : 1], f.cl.k, f.cl.order, f.cl.xmax, f.cl.ixmax,
-                                kinenergy(sp, p));
+        double fc = chebeval_n(as_global(f.cl.c[sp == PT_ELECTRON ? 0 : 1]), f.cl.k, f.cl.order, f.cl.xmax,
+                                f.cl.ixmax, kinenergy(sp, p));
         double sc = -fc / norm3(p);
         F = V3{F.x + p.x * sc, F.y + p.y * sc, F.z + p.z * sc};
     }
@@ -694,7 +715,8 @@ __device__ __forceinline__ V3 force(int sp, const Field& f, const V3& x, const V
 }
 
 // advance_particle(::RK2Pusher, y, dt), src/pusher.jl:41-63
-__device__ __forceinline__ void push(int sp, const Field& f, V3& x, V3& p, double& t, double dt) {
+template <class FT>
+__device__ __forceinline__ void push(int sp, const FT& f, V3& x, V3& p, double& t, double dt) {
     V3 v1 = velocity(sp, p);
     V3 f1 = force(sp, f, x, p);
     double dt2 = 2.0 * dt;
