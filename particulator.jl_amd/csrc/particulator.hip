@@ -140,6 +140,10 @@ struct Params {
 // Per-species pointers, table metadata and process records are staged in LDS once per
 // workgroup; the Chebyshev rate tables (and the rate bounds of a single swarm) follow in
 // dynamic LDS.
+// zeros after each species' rate table in LDS: the selection scan reads up to three
+// processes past the last one
+constexpr int SCAN_PAD = 9;
+
 template <typename T>
 using gptr = T PT_GLOBAL*;
 struct SpLite {
@@ -211,7 +215,7 @@ __device__ __forceinline__ void stage(const Params& P, double* lds, Shared& sh) 
         for (int q = threadIdx.x; q < tb.nprocs; q += blockDim.x) sh.procs[s][q] = tb.procs[q];
         int nr = (tb.k + 1) * tb.nprocs * 3;
         double* dst = lds + P.lds_rate_off[s];
-        for (int q = threadIdx.x; q < nr; q += blockDim.x) dst[q] = tb.rate[q];
+        for (int q = threadIdx.x; q < nr + SCAN_PAD; q += blockDim.x) dst[q] = q < nr ? tb.rate[q] : 0.0;
         if (P.rb_in_lds) {
             int nb = (tb.k + 1) * 3;
             double* d2 = lds + P.lds_rb_off[s];
@@ -230,6 +234,32 @@ __device__ __forceinline__ double setr(const SpLite& L, double E, int swarm, boo
     oob |= o;
     const size_t off = (size_t)(L.nswarm > 1 ? swarm : 0) * (L.k + 1) * 3 + i * 3;
     return L.rb_lds ? cheb3(as_lds(L.rb) + off, t1, t2) : cheb3(as_global(L.rb) + off, t1, t2);
+}
+
+// The selection scan of do_one_collision! (src/collisions.jl:160-186): the first process j
+// whose rate exceeds what is left of xi after subtracting the rates of processes 0..j-1; -1
+// for a null collision, xi then holding the remainder.  Rates are evaluated four at a time,
+// branch-free within a group, so that their LDS reads overlap; a group may run past the
+// row's last process into the next row or the table's padding (SCAN_PAD), whose values are
+// never used.
+__device__ __forceinline__ int select_process(const PT_LDS double* rt, int np, double t1, double t2, double& xi) {
+    int sel = -1;
+    for (int j = 0; j < np; j += 4) {
+        double nu[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) nu[u] = cheb3(rt + 3 * (j + u), t1, t2);
+        bool found = false;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool valid = j + u < np;
+            const bool hit = valid && !found && nu[u] > xi;
+            sel = hit ? j + u : sel;
+            found = found || hit;
+            xi = (valid && !found) ? xi - nu[u] : xi;
+        }
+        if (found) break;
+    }
+    return sel;
 }
 
 __device__ void flush_counters(const Params& P, BlockCounters* bc) {
@@ -688,16 +718,8 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, char* w
                         int iv = precheb(E, L.k, L.xmax, L.ixmax, t1, t2, o);
                         oob |= o;
                         double xi = g.next() * r;
-                        const PT_LDS double* rt = as_lds(L.rate) + (size_t)iv * L.nprocs * 3;
-                        int sel = -1;
-                        for (int j = 0; j < L.nprocs; j++) {
-                            double nu = cheb3(rt + 3 * j, t1, t2);
-                            if (nu > xi) {
-                                sel = j;
-                                break;
-                            }
-                            xi -= nu;
-                        }
+                        const int sel = select_process(as_lds(L.rate) + (size_t)iv * L.nprocs * 3, L.nprocs, t1,
+                                                       t2, xi);
                         if (sel < 0) {
                             if (!(xi >= 0.0)) atomicAdd(&sh.bc.flags[FLAG_RATEBOUND], 1ull);
                             atomicAdd(&sh.bc.counts[sp * NCOUNT], 1ull);
@@ -1858,16 +1880,7 @@ __global__ void __launch_bounds__(256) k_dbg_select(const Params* __restrict__ P
     bool o;
     int iv = precheb(E[i], L.k, L.xmax, L.ixmax, t1, t2, o);
     double xi = u[i] * r;
-    const PT_LDS double* rt = as_lds(L.rate) + (size_t)iv * L.nprocs * 3;
-    int sel = -1;
-    for (int j = 0; j < L.nprocs; j++) {
-        double nu = cheb3(rt + 3 * j, t1, t2);
-        if (nu > xi) {
-            sel = j;
-            break;
-        }
-        xi -= nu;
-    }
+    int sel = select_process(as_lds(L.rate) + (size_t)iv * L.nprocs * 3, L.nprocs, t1, t2, xi);
     interval[i] = o ? -1 : iv;
     proc[i] = sel;
     rbound[i] = r;
@@ -2065,7 +2078,7 @@ static int sync_params(pt_ctx* c) {
     c->hp.rb_in_lds = c->cfg.nswarm <= 1 ? 1 : 0;
     for (int s = 0; s < PT_NSPECIES; s++) {
         c->hp.lds_rate_off[s] = off;
-        if (c->hp.tab[s].valid) off += (c->hp.tab[s].k + 1) * c->hp.tab[s].nprocs * 3;
+        if (c->hp.tab[s].valid) off += (c->hp.tab[s].k + 1) * c->hp.tab[s].nprocs * 3 + SCAN_PAD;
         c->hp.lds_rb_off[s] = off;
         if (c->hp.tab[s].valid && c->hp.rb_in_lds) off += (c->hp.tab[s].k + 1) * 3;
     }
