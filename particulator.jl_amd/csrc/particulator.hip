@@ -361,7 +361,7 @@ __device__ __forceinline__ ull append(const Params& P, Shared& sh, int sp2, cons
 //
 // Work items q index the concatenation of the three species' item lists: slot ranges
 // [rb[s], re[s]) (SRC_RANGE) or the entries of the species' input work queues
-// (SRC_QUEUE).  A wave claims chunks of CHUNK consecutive items from `cursor` and hands
+// (SRC_QUEUE).  A wave claims chunks of consecutive items from `cursor` and hands
 // them out in order to its free lanes (ballot rank), so the particles a wave holds lie in
 // a short window of consecutive slots: refills read whole cache lines and neighbouring
 // write-backs meet in L2.
@@ -374,7 +374,10 @@ __device__ __forceinline__ ull append(const Params& P, Shared& sh, int sp2, cons
 // stream position and queued for that sampler kind (wave-aggregated appends), and the
 // lane moves on.  The sampling kernel then runs each queued kind with whole waves of the
 // same kind (no divergence between samplers; the learned samplers need it).
-constexpr int CHUNK = 256;
+// items a wave claims at a time: 128 in workgroups of up to 256 threads, 256 in larger ones
+// (the faster size for each launch shape the autotuner picks: the avalanche's 256-thread
+// and the sweep's 512-thread workgroups)
+__device__ __forceinline__ ull claim_chunk() { return blockDim.x > 256 ? 256ull : 128ull; }
 enum { SRC_RANGE = 0, SRC_QUEUE = 1 };
 
 struct Loaded {
@@ -548,6 +551,7 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, char* w
     const int lane = threadIdx.x & 63;
     const uint64_t ltmask = lane ? (~0ull >> (64 - lane)) : 0ull;
     // wave-uniform chunk state
+    const ull chunk = claim_chunk();
     ull cbeg = 0, cend = 0;
     bool drained = false;
     // lane state
@@ -601,14 +605,14 @@ __device__ __forceinline__ void select_loop(const Params& P, Shared& sh, char* w
             if (cbeg >= cend) {
                 int fl = __ffsll((unsigned long long)__ballot(1)) - 1;
                 ull b = 0;
-                if (lane == fl) b = global_add(cursor, (ull)CHUNK);
+                if (lane == fl) b = global_add(cursor, chunk);
                 b = __shfl(b, fl);
                 if (b >= total) {
                     drained = true;
                     break;
                 }
                 cbeg = b;
-                cend = b + CHUNK < total ? b + CHUNK : total;
+                cend = b + chunk < total ? b + chunk : total;
             }
             ull q = cbeg + (ull)__popcll(need & ltmask);
             ull cnt = (ull)__popcll(need);
