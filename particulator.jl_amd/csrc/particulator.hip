@@ -2107,7 +2107,7 @@ template <class T>
 struct Scratch {
     T* p = nullptr;
     ~Scratch() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
 };
 
@@ -2147,16 +2147,16 @@ const char* pt_last_error(const pt_ctx* ctx) { return ctx ? ctx->err.c_str() : p
 
 void pt_destroy(pt_ctx* c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& g : c->gexec)
-        if (g) hipGraphExecDestroy(g);
-    for (void* p : c->allocs) hipFree(p);
+        if (g) (void)hipGraphExecDestroy(g);
+    for (void* p : c->allocs) (void)hipFree(p);
     for (auto& e : c->ev)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
     for (auto& e : c->evq)
-        if (e) hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -2925,12 +2925,12 @@ static int capture(pt_ctx* c, int which, int steps, bool compact) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (lrc) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return lrc;
     }
     if (e != hipSuccess) return fail(c, PT_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(&c->gexec[which], g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(c, PT_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
     c->gsteps[which] = steps;
     c->glaunch[which] = c->launch;
