@@ -290,4 +290,113 @@ function run!(m::DeviceMultiPopulation, pusher, tfinal, dt, callback=VoidCallbac
     end
 end
 
+# ---- population statistics and read-back (src/population.jl:81-223, src/callback.jl:118-141)
+struct PtPopStats
+    n::UInt64
+    nactive::UInt64
+    weight::Float64
+    mean_energy::Float64
+    centroid::NTuple{3,Float64}
+    spread::Float64
+    max_energy::Float64
+end
+
+function stats(m::DeviceMultiPopulation, species)
+    out = Ref{PtPopStats}()
+    check(m.ctx, ccall((:pt_stats, libpt), Cint, (Ptr{Cvoid}, Cint, Ref{PtPopStats}), m.ctx, species, out))
+    return out[]
+end
+nactives(m::DeviceMultiPopulation, species) = Int(stats(m, species).nactive)
+weight(m::DeviceMultiPopulation, species) = stats(m, species).weight
+meanenergy(m::DeviceMultiPopulation, species) = stats(m, species).mean_energy
+maxenergy(m::DeviceMultiPopulation, species) = stats(m, species).max_energy
+spread(m::DeviceMultiPopulation, species) = (s = stats(m, species); (collect(s.centroid), s.spread))
+
+"""eachparticle (src/population.jl:81): the active particles of a species as a named tuple
+of columns (x, y, z, px, py, pz, w, t, s, r, uid, swarm)."""
+function download(m::DeviceMultiPopulation, species)
+    cap = Ref{UInt64}(0)
+    check(m.ctx, ccall((:pt_count, libpt), Cint, (Ptr{Cvoid}, Cint, Ref{UInt64}), m.ctx, species, cap))
+    k = max(Int(cap[]), 1)
+    cols = [zeros(Float64, k) for _ in 1:10]
+    act = zeros(UInt8, k)
+    uid = zeros(UInt64, k)
+    sw = zeros(Int32, k)
+    got = Ref{Csize_t}(0)
+    ptrs = [pointer(c) for c in cols]
+    GC.@preserve cols check(m.ctx, ccall((:pt_download, libpt), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Ptr{Float64}}, Ptr{UInt8}, Ptr{UInt64}, Ptr{Int32}, Csize_t, Ref{Csize_t}),
+        m.ctx, species, ptrs, act, uid, sw, k, got))
+    sel = findall(!iszero, act[1:Int(got[])])
+    names = (:x, :y, :z, :px, :py, :pz, :w, :t, :s, :r)
+    return merge(NamedTuple{names}(Tuple(c[sel] for c in cols)), (uid = uid[sel], swarm = sw[sel]))
+end
+
+"""CollisionCounter: null events, then one count per process in table order."""
+function collision_counts(m::DeviceMultiPopulation, species; reset=false, nmax=25)
+    out = zeros(Int64, nmax)
+    check(m.ctx, ccall((:pt_collision_counts, libpt), Cint, (Ptr{Cvoid}, Cint, Ptr{Int64}, Cint, Cint),
+                       m.ctx, species, out, nmax, reset))
+    return out
+end
+
+"""n particles generated on the device: kinetic energy e1 (or log-uniform in [e1, e2]),
+isotropic, at the origin."""
+init_particles!(m::DeviceMultiPopulation, species, n, e1, e2=e1; loguniform=false, swarm=0, uid0=0, t0=0.0) =
+    check(m.ctx, ccall((:pt_init_particles, libpt), Cint,
+                       (Ptr{Cvoid}, Cint, Csize_t, Cdouble, Cdouble, Cint, Cint, UInt64, Cdouble),
+                       m.ctx, species, n, e1, e2, loguniform, swarm, uid0, t0))
+
+"""n particles of kinetic energy e at the origin along dir (scripts/beam.jl:52-60)."""
+init_beam!(m::DeviceMultiPopulation, species, n, e, dir=(1e-6, 0.0, 1.0); swarm=0, uid0=0, t0=0.0) =
+    check(m.ctx, ccall((:pt_init_beam, libpt), Cint,
+                       (Ptr{Cvoid}, Cint, Csize_t, Cdouble, Ptr{Float64}, Cint, UInt64, Cdouble),
+                       m.ctx, species, n, e, Float64[dir...], swarm, uid0, t0))
+
+"""nsteps steps of dt from t0 with a host loop in the library (compaction every
+compact_every steps); returns the final time."""
+function run_steps!(m::DeviceMultiPopulation, t0, dt, nsteps; compact_every=8, check_every=16)
+    check(m.ctx, ccall((:pt_run, libpt), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cint, Cint, Cint),
+                       m.ctx, t0, dt, nsteps, compact_every, check_every))
+    return t0 + nsteps * dt
+end
+
+struct PtTuneTrial
+    launch::PtLaunch
+    rate::Float64
+end
+
+"""Time launch shapes, collision rounds and compaction cadences on the live run (the trial
+steps advance it from t0) and keep the fastest; returns (best launch, trials, time reached)."""
+function autotune!(m::DeviceMultiPopulation, t0, dt; trial_steps=4, max_trials=32)
+    trials = Vector{PtTuneTrial}(undef, max_trials)
+    nt = Ref{Cint}(0)
+    best = Ref{PtLaunch}()
+    tend = Ref{Cdouble}(0.0)
+    check(m.ctx, ccall((:pt_autotune, libpt), Cint,
+                       (Ptr{Cvoid}, Cdouble, Cdouble, Cint, Ptr{PtTuneTrial}, Cint, Ref{Cint}, Ref{PtLaunch},
+                        Ref{Cdouble}), m.ctx, t0, dt, trial_steps, trials, max_trials, nt, best, tend))
+    return best[], trials[1:min(Int(nt[]), max_trials)], tend[]
+end
+
+"""Depth-dose tally: (histogram, underflow, overflow, total)."""
+function dose(m::DeviceMultiPopulation, nbins; reset=false)
+    h = zeros(Float64, max(nbins, 1))
+    u, o, tot = Ref{Cdouble}(0), Ref{Cdouble}(0), Ref{Cdouble}(0)
+    check(m.ctx, ccall((:pt_dose_read, libpt), Cint,
+                       (Ptr{Cvoid}, Ptr{Float64}, Cint, Ref{Cdouble}, Ref{Cdouble}, Ref{Cdouble}, Cint),
+                       m.ctx, h, nbins, u, o, tot, reset))
+    return h[1:nbins], u[], o[], tot[]
+end
+
+"""Birth tally of a species: (weight histogram, count, weight)."""
+function tally(m::DeviceMultiPopulation, species, nbins)
+    h = zeros(Float64, max(nbins, 1))
+    c, w = Ref{Cdouble}(0), Ref{Cdouble}(0)
+    check(m.ctx, ccall((:pt_tally_read, libpt), Cint,
+                       (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ref{Cdouble}, Ref{Cdouble}),
+                       m.ctx, species, h, nbins, c, w))
+    return h[1:nbins], c[], w[]
+end
+
 end
