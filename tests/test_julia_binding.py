@@ -1,0 +1,98 @@
+"""The Julia binding (particulator.jl_amd/julia/ParticulatorHIP.jl) cannot run here (no Julia
+toolchain), so its struct mirrors are checked against the C header: every `struct Pt...`
+is parsed, laid out with the C rules Julia uses for isbits structs, and each field's offset
+and the struct's size compared with offsetof/sizeof from a C program compiled against
+include/particulator_hip.h.  Every entry point the binding calls must be declared there."""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+JL = os.path.join(ROOT, "particulator.jl_amd", "julia", "ParticulatorHIP.jl")
+HDR = os.path.join(ROOT, "include", "particulator_hip.h")
+
+PRIM = {"Int32": 4, "UInt32": 4, "Cint": 4, "Int64": 8, "UInt64": 8, "Float64": 8, "Cdouble": 8,
+        "Csize_t": 8}
+
+
+def c_name(jl):
+    """PtFieldDesc -> pt_field_desc"""
+    return "pt_" + re.sub(r"(?<!^)([A-Z])", r"_\1", jl[2:]).lower()
+
+
+def parse_structs(src):
+    out = {}
+    for m in re.finditer(r"^struct (Pt\w+)\n(.*?)^end", src, re.S | re.M):
+        fields = []
+        for line in m.group(2).splitlines():
+            for decl in line.split(";"):
+                decl = decl.strip()
+                if decl:
+                    name, typ = decl.split("::")
+                    fields.append((name.strip(), typ.strip()))
+        out[m.group(1)] = fields
+    return out
+
+
+def layout(structs, name):
+    """(size, align, {field: offset}) of a struct under C layout rules"""
+    off, align, offs = 0, 1, {}
+    for f, t in structs[name]:
+        s, a = type_size(structs, t)
+        off = (off + a - 1) // a * a
+        offs[f] = off
+        off += s
+        align = max(align, a)
+    return (off + align - 1) // align * align, align, offs
+
+
+def type_size(structs, t):
+    t = t.replace(" ", "")
+    if t in PRIM:
+        return PRIM[t], PRIM[t]
+    if t.startswith("Ptr{"):
+        return 8, 8
+    m = re.fullmatch(r"NTuple\{(\d+),(\w+)\}", t)
+    if m:
+        s, a = type_size(structs, m.group(2))
+        return int(m.group(1)) * s, a
+    if t in structs:
+        s, a, _ = layout(structs, t)
+        return s, a
+    raise AssertionError("unknown field type %s" % t)
+
+
+def test_struct_mirrors_match_the_header(tmp_path):
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    structs = parse_structs(open(JL).read())
+    assert len(structs) >= 10
+    lines, expect = [], []
+    for jl, fields in sorted(structs.items()):
+        cn = c_name(jl)
+        size, _, offs = layout(structs, jl)
+        lines.append('printf("%%s %%zu\\n", "%s", sizeof(%s));' % (jl, cn))
+        expect.append("%s %d" % (jl, size))
+        for f, _ in fields:
+            lines.append('printf("%%s.%%s %%zu\\n", "%s", "%s", offsetof(%s, %s));' % (jl, f, cn, f))
+            expect.append("%s.%s %d" % (jl, f, offs[f]))
+    prog = tmp_path / "layout.c"
+    prog.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "particulator_hip.h"\n'
+                    "int main(void) {\n" + "\n".join(lines) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HDR), "-o", str(exe), str(prog)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    assert got[:len(expect)] == expect
+
+
+def test_binding_calls_declared_entry_points():
+    hdr = open(HDR).read()
+    called = set(re.findall(r"ccall\(\(:(pt_\w+)", open(JL).read()))
+    assert len(called) >= 25
+    missing = [f for f in sorted(called) if not re.search(r"\b%s\(" % f, hdr)]
+    assert not missing, missing
