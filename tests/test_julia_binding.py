@@ -2,7 +2,8 @@
 toolchain), so its struct mirrors are checked against the C header: every `struct Pt...`
 is parsed, laid out with the C rules Julia uses for isbits structs, and each field's offset
 and the struct's size compared with offsetof/sizeof from a C program compiled against
-include/particulator_hip.h.  Every entry point the binding calls must be declared there."""
+include/particulator_hip.h.  Every entry point the binding calls must be declared there,
+with the same number of arguments of the same widths."""
 import os
 import re
 import shutil
@@ -96,3 +97,37 @@ def test_binding_calls_declared_entry_points():
     assert len(called) >= 25
     missing = [f for f in sorted(called) if not re.search(r"\b%s\(" % f, hdr)]
     assert not missing, missing
+
+
+def _c_class(param):
+    p = param.strip()
+    if "*" in p or "[" in p:
+        return "ptr"
+    base = p.split()[-2] if len(p.split()) > 1 else p
+    return {"int": "i32", "int32_t": "i32", "uint32_t": "u32", "double": "f64", "float": "f32", "size_t": "u64",
+            "uint64_t": "u64", "int64_t": "i64", "void": "void"}[base]
+
+
+def _jl_class(t):
+    t = t.strip()
+    if t.startswith(("Ptr{", "Ref{")) or t == "Cstring":
+        return "ptr"
+    return {"Cint": "i32", "Int32": "i32", "UInt32": "u32", "Cdouble": "f64", "Float64": "f64", "Cfloat": "f32",
+            "Csize_t": "u64", "UInt64": "u64", "Int64": "i64", "Cvoid": "void"}[t]
+
+
+def test_binding_ccall_signatures_match_the_header():
+    """argument count and width of every ccall against the header's prototype"""
+    hdr = re.sub(r"/\*.*?\*/", "", open(HDR).read(), flags=re.S)
+    protos = {}
+    for m in re.finditer(r"\b(int|void|const char\s*\*)\s*(pt_\w+)\(([^;]*?)\);", hdr, re.S):
+        params = [p for p in m.group(3).replace("\n", " ").split(",") if p.strip() and p.strip() != "void"]
+        ret = "ptr" if "char" in m.group(1) else ("void" if m.group(1) == "void" else "i32")
+        protos[m.group(2)] = (ret, [_c_class(p) for p in params])
+    calls = re.findall(r"ccall\(\(:(pt_\w+),\s*libpt\),\s*(\w+),\s*\(([^()]*)\)", open(JL).read())
+    assert len(calls) >= 30
+    for name, ret, types in calls:
+        want_ret, want = protos[name]
+        got = [_jl_class(t) for t in types.replace("\n", " ").split(",") if t.strip()]
+        assert _jl_class(ret) == want_ret, (name, ret)
+        assert got == want, (name, got, want)
