@@ -137,15 +137,17 @@ struct Params {
 };
 
 // ------------------------------------------------------------------ LDS state
-// Per-species pointers, table metadata and process records are staged in LDS once per
-// workgroup; the Chebyshev rate tables (and the rate bounds of a single swarm) follow in
-// dynamic LDS.
 // zeros after each species' rate table in LDS: the selection scan reads up to three
 // processes past the last one
 constexpr int SCAN_PAD = 9;
 
 template <typename T>
 using gptr = T PT_GLOBAL*;
+
+// Per-species pointers, table metadata and process records are staged in LDS once per
+// workgroup; the Chebyshev rate tables (and the rate bounds of a single swarm) follow in
+// dynamic LDS.  The particle columns are typed as global memory and the tables are read
+// through LDS-typed pointers, so that neither becomes a flat access.
 struct SpLite {
     gptr<double> f[PT_NFIELDS];
     gptr<uint8_t> active;
