@@ -1,6 +1,7 @@
 /* A host of the C ABI with no Python: the runaway-electron swarm in dry air built from raw
  * processes (pt_sb_build, pt_table_build, pt_process_desc), uploaded, stepped and reported
- * as one line of JSON.  Usage: swarm <brem_SB dir> <n> <steps> <seed> */
+ * as one line of JSON.  Usage: swarm <brem_SB dir> <n> <steps> <seed> [graph]
+ * (graph: the steps run as replays of a captured graph, pt_run_graph, instead of pt_run) */
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -55,9 +56,10 @@ static void read_sb(const char* dir, int Z, int ncum, sb_t* s) {
 
 int main(int argc, char** argv) {
     if (argc < 5) {
-        fprintf(stderr, "usage: swarm <brem_SB dir> <n> <steps> <seed>\n");
+        fprintf(stderr, "usage: swarm <brem_SB dir> <n> <steps> <seed> [graph]\n");
         return 2;
     }
+    const int graph = argc > 5 && strcmp(argv[5], "graph") == 0;
     const int ncum = 1000, n = atoi(argv[2]), steps = atoi(argv[3]);
     const uint64_t seed = strtoull(argv[4], NULL, 10);
     const double cut = 1e4 * EV, dt = 1e-12, efield = 1e6;
@@ -160,7 +162,10 @@ int main(int argc, char** argv) {
     CHECK(ctx, pt_set_field(ctx, 0, &fd));
     CHECK(ctx, pt_init_particles(ctx, PT_ELECTRON, (size_t)n, 1e6 * EV, 1e6 * EV, 0, 0, 0, 0.0));
     CHECK(ctx, pt_init(ctx));
-    CHECK(ctx, pt_run(ctx, 0.0, dt, steps, 8, steps));
+    if (graph)
+        CHECK(ctx, pt_run_graph(ctx, 0.0, dt, steps, steps));
+    else
+        CHECK(ctx, pt_run(ctx, 0.0, dt, steps, 8, steps));
     pt_pop_stats st;
     CHECK(ctx, pt_stats(ctx, PT_ELECTRON, &st));
     printf("{\"nactive\": %llu, \"weight\": %.17g, \"mean_energy\": %.17g, \"max_energy\": %.17g}\n",

@@ -54,3 +54,12 @@ def test_c_host_matches_python_host(tmp_path):
     for k in ("nactive", "weight", "max_energy"):
         assert c[k] == p[k], k
     assert abs(c["mean_energy"] - p["mean_energy"]) <= 1e-12 * p["mean_energy"]
+    # the same run as replays of a captured graph (pt_run_graph)
+    print("C host, captured graph", flush=True)
+    r = subprocess.run([exe, SB_DIR, str(n), str(steps), str(seed), "graph"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    g = json.loads(r.stdout.strip().splitlines()[-1])
+    for k in ("nactive", "weight", "max_energy"):
+        assert g[k] == c[k], k
+    assert abs(g["mean_energy"] - c["mean_energy"]) <= 1e-12 * c["mean_energy"]
